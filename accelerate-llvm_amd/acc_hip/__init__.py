@@ -1,0 +1,330 @@
+"""ctypes binding of libaccelerate_hip.so (the C ABI in include/accelerate_hip.h).
+
+This is the binding a non-C host (here: the tests, the benchmark and the
+serving loop) uses; a Haskell backend binds the same symbols with
+`foreign import ccall`.  Every skeleton call goes to the gfx950 kernels in the
+shared library; there is no CPU fallback: if the library or a GPU is missing,
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int64, c_size_t, c_uint, c_void_p
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libaccelerate_hip.so")
+
+# acc_status
+SUCCESS, INVALID_VALUE, OUT_OF_MEMORY, HIP_ERROR, UNSUPPORTED, EMPTY_ARRAY, NOT_READY, NO_DEVICE = range(8)
+# acc_type
+INT32, INT64, WORD32, WORD64, FLOAT32, FLOAT64 = range(6)
+# acc_op
+ADD, MUL, MIN, MAX, AND, OR, XOR, SUB, EXCHANGE = range(9)
+# acc_direction
+LEFT_TO_RIGHT, RIGHT_TO_LEFT = 0, 1
+# acc_boundary
+CLAMP, MIRROR, WRAP, CONSTANT = range(4)
+
+_NP_TO_ACC = {
+    np.dtype(np.int32): INT32,
+    np.dtype(np.int64): INT64,
+    np.dtype(np.uint32): WORD32,
+    np.dtype(np.uint64): WORD64,
+    np.dtype(np.float32): FLOAT32,
+    np.dtype(np.float64): FLOAT64,
+}
+
+
+class AccError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[acc status {status}] {message}")
+        self.status = status
+
+
+class OutOfMemory(AccError):
+    pass
+
+
+class EmptyArray(AccError):
+    pass
+
+
+_lib = None
+
+
+class DeviceProps(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("compute_units", c_int),
+        ("wave_size", c_int),
+        ("max_threads_per_block", c_int),
+        ("lds_per_block", c_size_t),
+        ("total_global_mem", c_size_t),
+        ("clock_khz", c_int),
+        ("memory_clock_khz", c_int),
+        ("memory_bus_width", c_int),
+        ("l2_cache_bytes", c_int),
+    ]
+
+
+# exported symbol -> (restype, argtypes); every declaration in the header
+SIGNATURES = {
+    "acc_hip_last_error": (c_char_p, []),
+    "acc_hip_abi_version": (c_int, []),
+    "acc_hip_device_count": (c_int, [POINTER(c_int)]),
+    "acc_hip_device_props": (c_int, [c_int, POINTER(DeviceProps)]),
+    "acc_hip_device_synchronize": (c_int, [c_int]),
+    "acc_hip_malloc": (c_int, [c_int, c_size_t, POINTER(c_void_p)]),
+    "acc_hip_free": (c_int, [c_int, c_void_p]),
+    "acc_hip_mem_get_info": (c_int, [c_int, POINTER(c_size_t), POINTER(c_size_t)]),
+    "acc_hip_set_alloc_limit": (c_int, [c_int, c_size_t]),
+    "acc_hip_allocated_bytes": (c_int, [c_int, POINTER(c_size_t)]),
+    "acc_hip_host_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
+    "acc_hip_host_free": (c_int, [c_void_p]),
+    "acc_hip_memcpy_h2d_async": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "acc_hip_memcpy_d2h_async": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "acc_hip_memcpy_d2d_async": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "acc_hip_fill_async": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t]),
+    "acc_hip_stream_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "acc_hip_stream_destroy": (c_int, [c_void_p]),
+    "acc_hip_stream_synchronize": (c_int, [c_void_p]),
+    "acc_hip_stream_query": (c_int, [c_void_p]),
+    "acc_hip_stream_device": (c_int, [c_void_p, POINTER(c_int)]),
+    "acc_hip_stream_reserve": (c_int, [c_void_p, c_size_t]),
+    "acc_hip_event_create": (c_int, [c_int, c_int, POINTER(c_void_p)]),
+    "acc_hip_event_destroy": (c_int, [c_void_p]),
+    "acc_hip_event_record": (c_int, [c_void_p, c_void_p]),
+    "acc_hip_stream_wait_event": (c_int, [c_void_p, c_void_p]),
+    "acc_hip_event_query": (c_int, [c_void_p]),
+    "acc_hip_event_synchronize": (c_int, [c_void_p]),
+    "acc_hip_event_elapsed_ms": (c_int, [c_void_p, c_void_p, POINTER(c_float)]),
+    "acc_hip_module_load_data": (c_int, [c_int, c_void_p, c_size_t, POINTER(c_void_p)]),
+    "acc_hip_module_unload": (c_int, [c_void_p]),
+    "acc_hip_module_get_function": (c_int, [c_void_p, c_char_p, POINTER(c_void_p)]),
+    "acc_hip_function_attributes": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "acc_hip_launch": (c_int, [c_void_p, c_uint, c_uint, c_uint, c_uint, c_uint, c_uint, c_uint, c_void_p,
+                               POINTER(c_void_p)]),
+    "acc_hip_graph_begin_capture": (c_int, [c_void_p]),
+    "acc_hip_graph_end_capture": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "acc_hip_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "acc_hip_graph_destroy": (c_int, [c_void_p]),
+    "acc_hip_fold": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
+    "acc_hip_fold_seg": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p,
+                                 c_int64, c_void_p]),
+    "acc_hip_scan": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
+    "acc_hip_scan_prime": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
+                                   c_void_p]),
+    "acc_hip_permute": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int,
+                                c_void_p, c_int64]),
+    "acc_hip_stencil2d": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int64,
+                                  c_int64, c_void_p]),
+}
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C accelerate-llvm_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status == SUCCESS:
+        return
+    msg = lib().acc_hip_last_error().decode(errors="replace")
+    if status == OUT_OF_MEMORY:
+        raise OutOfMemory(status, msg)
+    if status == EMPTY_ARRAY:
+        raise EmptyArray(status, msg)
+    raise AccError(status, msg)
+
+
+def acc_type(dtype) -> int:
+    try:
+        return _NP_TO_ACC[np.dtype(dtype)]
+    except KeyError:
+        raise TypeError(f"unsupported element type {dtype}") from None
+
+
+def device_count() -> int:
+    n = c_int(0)
+    check(lib().acc_hip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_props(device: int = 0) -> DeviceProps:
+    p = DeviceProps()
+    check(lib().acc_hip_device_props(device, ctypes.byref(p)))
+    return p
+
+
+class Stream:
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = c_void_p()
+        check(lib().acc_hip_stream_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def synchronize(self) -> None:
+        check(lib().acc_hip_stream_synchronize(self.handle))
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().acc_hip_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self, device: int = 0, timing: bool = True):
+        h = c_void_p()
+        check(lib().acc_hip_event_create(device, 1 if timing else 0, ctypes.byref(h)))
+        self.handle = h
+
+    def record(self, stream: Stream) -> None:
+        check(lib().acc_hip_event_record(self.handle, stream.handle))
+
+    def synchronize(self) -> None:
+        check(lib().acc_hip_event_synchronize(self.handle))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = c_float(0)
+        check(lib().acc_hip_event_elapsed_ms(self.handle, later.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().acc_hip_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device buffer of `count` elements of `dtype` (one struct-of-arrays field)."""
+
+    def __init__(self, count: int, dtype, device: int = 0):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.device = device
+        self.nbytes = self.count * self.dtype.itemsize
+        p = c_void_p()
+        check(lib().acc_hip_malloc(device, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_numpy(cls, stream: Stream, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype, stream.device)
+        if a.size:
+            check(lib().acc_hip_memcpy_h2d_async(stream.handle, c_void_p(d.ptr), a.ctypes.data_as(c_void_p), a.nbytes))
+            stream.synchronize()
+        return d
+
+    def to_numpy(self, stream: Stream) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        if self.count:
+            check(lib().acc_hip_memcpy_d2h_async(stream.handle, out.ctypes.data_as(c_void_p), c_void_p(self.ptr),
+                                                 self.nbytes))
+            stream.synchronize()
+        return out
+
+    def view(self, offset_elems: int, count: int) -> "DeviceView":
+        return DeviceView(self, offset_elems, count)
+
+    def free(self) -> None:
+        if self.ptr:
+            check(lib().acc_hip_free(self.device, c_void_p(self.ptr)))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceView:
+    """A sub-range of a DeviceArray (not owning)."""
+
+    def __init__(self, base: DeviceArray, offset: int, count: int):
+        self.base = base
+        self.dtype = base.dtype
+        self.count = count
+        self.device = base.device
+        self.ptr = base.ptr + offset * base.dtype.itemsize
+        self.nbytes = count * base.dtype.itemsize
+
+    def to_numpy(self, stream: Stream) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        if self.count:
+            check(lib().acc_hip_memcpy_d2h_async(stream.handle, out.ctypes.data_as(c_void_p), c_void_p(self.ptr),
+                                                 self.nbytes))
+            stream.synchronize()
+        return out
+
+
+def _scalar(dtype, v):
+    if v is None:
+        return None, None
+    a = np.array([v], dtype=dtype)
+    return a, a.ctypes.data_as(c_void_p)
+
+
+def _p(x):
+    return c_void_p(x.ptr) if x is not None else None
+
+
+# ----------------------------------------------------------------- skeletons
+def fold(stream, op, seed, src, outer, inner, out):
+    keep, sp = _scalar(src.dtype, seed)
+    check(lib().acc_hip_fold(stream.handle, acc_type(src.dtype), op, sp, _p(src), outer, inner, _p(out)))
+
+
+def fold_seg(stream, op, seed, src, outer, inner, offsets, nseg, out):
+    keep, sp = _scalar(src.dtype, seed)
+    check(lib().acc_hip_fold_seg(stream.handle, acc_type(src.dtype), op, sp, _p(src), outer, inner,
+                                 acc_type(offsets.dtype), _p(offsets), nseg, _p(out)))
+
+
+def scan(stream, op, direction, seed, src, outer, inner, out):
+    keep, sp = _scalar(src.dtype, seed)
+    check(lib().acc_hip_scan(stream.handle, acc_type(src.dtype), op, direction, sp, _p(src), outer, inner, _p(out)))
+
+
+def scan_prime(stream, op, direction, seed, src, outer, inner, out, sums):
+    keep, sp = _scalar(src.dtype, seed)
+    check(lib().acc_hip_scan_prime(stream.handle, acc_type(src.dtype), op, direction, sp, _p(src), outer, inner,
+                                   _p(out), _p(sums)))
+
+
+def permute(stream, op, defaults, out, m, src, src_fill, index, n):
+    dt = out.dtype
+    keep, fp = _scalar(dt, src_fill)
+    check(lib().acc_hip_permute(stream.handle, acc_type(dt), op, _p(defaults), _p(out), m, _p(src), fp,
+                                acc_type(index.dtype), _p(index), n))
+
+
+def stencil2d(stream, ry, rx, weights, boundary, constant, src, height, width, out):
+    dt = src.dtype
+    w = np.ascontiguousarray(np.asarray(weights, dtype=dt).reshape(-1))
+    keep, cp = _scalar(dt, constant)
+    check(lib().acc_hip_stencil2d(stream.handle, acc_type(dt), ry, rx, w.ctypes.data_as(c_void_p), boundary, cp,
+                                  _p(src), height, width, _p(out)))

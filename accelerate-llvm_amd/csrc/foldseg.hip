@@ -1,0 +1,330 @@
+// foldSeg / fold1Seg, and fold along the innermost dimension of a rank >= 2
+// array (every row one segment), in ONE load-balanced launch.
+//
+// The reference assigns one block or one warp per segment
+// (PTX/CodeGen/FoldSeg.hs:79-264 and :274-458, selected at
+// PTX/Execute.hs:361-366) and one block per row for foldDim
+// (PTX/CodeGen/Fold.hs:296-405), so long segments serialise and short ones
+// idle most lanes.  Here the elements are cut into equal tiles regardless of
+// segment boundaries and the fold becomes a segmented scan over
+// (head-flag, value) pairs, evaluated in registers; the value at the last
+// element of each segment is its result.  A segment that crosses tiles takes
+// its carry-in from a decoupled look-back in which every tile that contains a
+// segment head publishes its final prefix immediately, so look-back depth is
+// at most the number of tiles a segment spans.
+//
+// Segments are given as offsets (the `scanl (+) 0` of the lengths; the
+// frontend has already applied it, PTX/Execute.hs:359,
+// NAT/CodeGen/FoldSeg.hs:99-101) or, for foldDim, as the uniform row length.
+// Empty segments yield the seed (PTX/CodeGen/FoldSeg.hs:170-178); fold1Seg
+// leaves them untouched (it requires non-empty segments,
+// NAT/CodeGen/FoldSeg.hs:96-97).
+//
+// HBM roofline: n*sizeof(T) + (m+1)*sizeof(I) + m*sizeof(T) bytes.
+#include "dispatch.hpp"
+#include "lookback.hpp"
+#include "vecmem.hpp"
+
+namespace acc {
+
+template <typename T> struct SegV {
+  T v;
+  uint32_t f;  // a segment starts at (or after) the first element
+};
+
+template <typename Op> struct SegOp {
+  Op op;
+  template <typename T>
+  __device__ __forceinline__ SegV<T> operator()(SegV<T> a, SegV<T> b) const {
+    SegV<T> r;
+    r.v = b.f ? b.v : op(a.v, b.v);
+    r.f = a.f | b.f;
+    return r;
+  }
+};
+
+// Segment starts given by an offsets array: start(g) = off[g], g in [0, S].
+template <typename I> struct OffsetStarts {
+  const I* off;
+  __device__ __forceinline__ int64_t start(int64_t g) const { return (int64_t)off[g]; }
+};
+
+// foldDim: start(g) = g * len.
+struct UniformStarts {
+  int64_t len;
+  __device__ __forceinline__ int64_t start(int64_t g) const { return g * len; }
+};
+
+template <typename T, typename Starts> struct SegArgs {
+  const T* in;
+  T* out;
+  Starts st;
+  int64_t nseg;    // S
+  int64_t e_begin; // start(0)
+  int64_t e_end;   // start(S)
+  int64_t tile0;   // first tile index (e_begin / TILE)
+  uint32_t ntiles;
+  uint32_t has_seed;
+  T seed;
+  uint32_t check_empty;  // look for empty segments (offsets only)
+  SyncCtl* ctl;
+  uint64_t* status;
+};
+
+template <typename Starts>
+__device__ __forceinline__ int64_t seg_search(const Starts& st, int64_t lo, int64_t hi, int64_t e) {
+  // last g in [lo, hi] with start(g) <= e; lo - 1 if none
+  int64_t a = lo, b = hi + 1;  // invariant: start(a-1) <= e (virtual), start(b) > e
+  while (a < b) {
+    int64_t m = (a + b) >> 1;
+    if (st.start(m) <= e) a = m + 1; else b = m;
+  }
+  return a - 1;
+}
+
+template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
+  constexpr int VEC = Vec<T>::N;
+  constexpr int W = BLOCK / 64;
+  constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
+  using S = SegV<T>;
+  constexpr int NW = Tagged<S>::NW;
+  SegOp<Op> sop{Op{}};
+  Op op{};
+  __shared__ uint32_t s_u32[2];
+  __shared__ int64_t s_range[2];
+  __shared__ int64_t s_starts[CAP];
+  __shared__ Maybe<S> s_wave[W];
+  __shared__ Maybe<S> s_prefix;
+
+  const uint32_t epoch = block_epoch(a.ctl, &s_u32[0]);
+  const uint32_t t = block_ticket(a.ctl, &s_u32[1]);
+  const int lane = lane_id();
+  const int w = threadIdx.x / 64;
+  const int64_t base = (a.tile0 + (int64_t)t) * TILE;
+
+  // ---- segment range of the tile, staged in LDS when it fits
+  // Segment g in [0, S) covers [start(g), start(g+1)); "segment" S collects
+  // the elements after the last one (start(S+1) = +inf) and elements before
+  // start(0) belong to "segment" -1.  Neither produces a result.
+  auto sv = [&](int64_t g) -> int64_t { return g > a.nseg ? INT64_MAX : a.st.start(g); };
+  if (threadIdx.x == 0) {
+    const int64_t lo_e = max(base, a.e_begin);
+    const int64_t hi_e = min(base + TILE, a.e_end) - 1;
+    s_range[0] = max((int64_t)0, seg_search(a.st, 0, a.nseg, lo_e));
+    s_range[1] = max((int64_t)0, seg_search(a.st, 0, a.nseg, hi_e));
+  }
+  __syncthreads();
+  const int64_t g_lo = s_range[0], g_hi = s_range[1];
+  const int64_t cnt = g_hi - g_lo + 2;  // starts g_lo .. g_hi+1
+  const bool staged = cnt <= CAP;
+  if (staged)
+    for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) s_starts[i] = sv(g_lo + i);
+  __syncthreads();
+  auto start_of = [&](int64_t g) -> int64_t {
+    return staged ? s_starts[g - g_lo] : sv(g);
+  };
+  auto search = [&](int64_t e) -> int64_t {
+    int64_t x = g_lo, y = g_hi + 1;
+    while (x < y) {
+      int64_t m = (x + y) >> 1;
+      if (start_of(m) <= e) x = m + 1; else y = m;
+    }
+    return x - 1;  // >= g_lo - 1
+  };
+
+  // ---- load and flag
+  S x[K][VEC];
+  int64_t g0[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
+    T v[VEC];
+    if (e0 >= a.e_begin && e0 + VEC <= a.e_end) {
+      Vec<T> r = load16_stream(a.in + e0);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v[j] = r.e[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const int64_t e = e0 + j;
+        v[j] = (e >= a.e_begin && e < a.e_end) ? a.in[e] : T();
+      }
+    }
+    int64_t g = search(e0);
+    g0[k] = g;
+    int64_t cur = g >= g_lo ? start_of(g) : INT64_MIN;
+    int64_t nxt = start_of(g + 1);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const int64_t e = e0 + j;
+      const bool valid = e >= a.e_begin && e < a.e_end;
+      while (nxt <= e && g < g_hi) { ++g; cur = nxt; nxt = start_of(g + 1); }
+      x[k][j].v = v[j];
+      x[k][j].f = (!valid || cur == e) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int j = 1; j < VEC; ++j) x[k][j] = sop(x[k][j - 1], x[k][j]);
+  }
+
+  // ---- segmented scan across lanes, chunks and waves
+  auto none = []() { Maybe<S> m; m.v.v = T(); m.v.f = 0; m.ok = 0; return m; };
+  Maybe<S> pre[K];
+  Maybe<S> run = none();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    S inc = wave_inclusive_scan(x[k][VEC - 1], sop);
+    S up = shift_up1(inc);
+    Maybe<S> exl = none();
+    if (lane > 0) { exl.v = up; exl.ok = 1; }
+    pre[k] = mcombine(sop, run, exl);
+    Maybe<S> tot{read_lane(inc, 63), 1u};
+    run = mcombine(sop, run, tot);
+  }
+  if (lane == 0) s_wave[w] = run;
+  __syncthreads();
+  Maybe<S> wp = none(), agg = none();
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const Maybe<S> v = s_wave[i];
+    if (i < w) wp = mcombine(sop, wp, v);
+    agg = mcombine(sop, agg, v);
+  }
+
+  // ---- carry-in from predecessor tiles
+  if (w == 0) {
+    Maybe<S> X = none();
+    uint64_t* slot = a.status + (size_t)t * NW;
+    const bool last = t + 1 == a.ntiles;
+    if (t == 0 || agg.v.f) {
+      // a head inside the tile makes its inclusive prefix final at once
+      if (lane == 0 && !last) Tagged<S>::publish(slot, agg.v, make_tag(epoch, ST_PREFIX));
+    } else if (lane == 0 && !last) {
+      Tagged<S>::publish(slot, agg.v, make_tag(epoch, ST_AGGREGATE));
+    }
+    // the first element needs a carry unless it starts a segment
+    const bool first_is_head = x[0][0].f != 0;  // lane 0, wave 0 holds element 0
+    const bool need = t > 0 && !__builtin_amdgcn_readfirstlane((int)first_is_head);
+    if (need) {
+      X.v = wave_lookback<S>(a.status, t, epoch, sop);
+      X.ok = 1;
+      if (!agg.v.f && lane == 0 && !last)
+        Tagged<S>::publish(slot, sop(X.v, agg.v), make_tag(epoch, ST_PREFIX));
+    }
+    if (lane == 0) s_prefix = X;
+  }
+  __syncthreads();
+  const Maybe<S> XW = mcombine(sop, s_prefix, wp);
+
+  // ---- results at segment ends
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const Maybe<S> P = mcombine(sop, XW, pre[k]);
+    const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
+    int64_t g = g0[k];
+    int64_t nxt = start_of(g + 1);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const int64_t e = e0 + j;
+      if (e < a.e_begin || e >= a.e_end) continue;
+      while (nxt <= e && g < g_hi) { ++g; nxt = start_of(g + 1); }
+      if (nxt == e + 1 && g >= 0 && g < a.nseg) {
+        T r = P.ok ? sop(P.v, x[k][j]).v : x[k][j].v;
+        if (a.has_seed) r = op(a.seed, r);  // seed on the left
+        a.out[g] = r;
+      }
+    }
+  }
+
+  // ---- empty segments (their value is the seed)
+  if (a.check_empty && a.has_seed) {
+    const int64_t per = (a.nseg + a.ntiles - 1) / a.ntiles;
+    const int64_t s0 = (int64_t)t * per, s1 = min(s0 + per, a.nseg);
+    for (int64_t g = s0 + threadIdx.x; g < s1; g += BLOCK)
+      if (a.st.start(g) == a.st.start(g + 1)) a.out[g] = a.seed;
+  }
+  block_retire(a.ctl, gridDim.x);
+}
+
+template <typename T> struct SegTune {
+  static constexpr int BLOCK = 256;
+  static constexpr int K = 4;
+  static constexpr int CAP = 1024;
+};
+
+template <typename T, typename Op, typename Starts>
+int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg, int64_t e_begin,
+                    int64_t e_end, const T* seed, bool check_empty) {
+  constexpr int BLOCK = SegTune<T>::BLOCK, K = SegTune<T>::K, CAP = SegTune<T>::CAP;
+  constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
+  constexpr int NW = Tagged<SegV<T>>::NW;
+  if ((uintptr_t)in % 16) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: input must be 16-byte aligned");
+  if (e_end < e_begin) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: decreasing offsets");
+  const int64_t tile0 = e_begin / TILE;
+  int64_t ntiles = e_end > e_begin ? ceil_div(e_end, TILE) - tile0 : 0;
+  if (ntiles == 0) {
+    // no elements: every segment is empty
+    if (!seed || !check_empty) return ACC_SUCCESS;
+    return acc_hip_fill_async(s, out, seed, sizeof(T), (size_t)nseg);
+  }
+  if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: too many elements");
+  SyncCtl* ctl;
+  uint64_t* status;
+  ACC_CHECK(stream_workspace(s, (size_t)ntiles * NW * 8, &ctl, &status));
+  SegArgs<T, Starts> a{in, out, st, nseg, e_begin, e_end, tile0, (uint32_t)ntiles, seed ? 1u : 0u,
+                       seed ? *seed : T(), check_empty ? 1u : 0u, ctl, status};
+  hipLaunchKernelGGL((k_fold_seg<T, Op, Starts, BLOCK, K, CAP>), dim3((unsigned)ntiles), dim3(BLOCK), 0,
+                     s->stream, a);
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
+
+int fold_dim(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void* in, int64_t outer,
+             int64_t inner, void* out) {
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    return with_op<T>(op, [&](auto f) {
+      using Op = decltype(f);
+      return fold_seg_launch<T, Op, UniformStarts>(s, (const T*)in, (T*)out, UniformStarts{inner}, outer, 0,
+                                                   outer * inner, (const T*)seed, false);
+    });
+  });
+}
+
+}  // namespace acc
+
+using namespace acc;
+
+extern "C" int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void* in,
+                                int64_t outer, int64_t inner, acc_type offset_type, const void* offsets,
+                                int64_t nseg, void* out) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: null stream");
+  if (outer < 0 || inner < 0 || nseg < 0) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: negative extent");
+  if (offset_type != ACC_INT32 && offset_type != ACC_INT64)
+    return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: offsets must be Int32 or Int64");
+  if (outer == 0 || nseg == 0) return ACC_SUCCESS;
+  if (!in || !out || !offsets) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: null array");
+  const size_t B = type_size(t);
+  if (!B) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: element type %d", (int)t);
+  ACC_CHECK(set_device(s->device));
+  const size_t I = offset_type == ACC_INT32 ? 4 : 8;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    return with_op<T>(op, [&](auto f) {
+      using Op = decltype(f);
+      for (int64_t r = 0; r < outer; ++r) {
+        const T* rin = (const T*)in + r * inner;
+        T* rout = (T*)out + r * nseg;
+        int rc;
+        if (I == 4)
+          rc = fold_seg_launch<T, Op, OffsetStarts<int32_t>>(s, rin, rout, OffsetStarts<int32_t>{(const int32_t*)offsets},
+                                                             nseg, 0, inner, (const T*)seed, true);
+        else
+          rc = fold_seg_launch<T, Op, OffsetStarts<int64_t>>(s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets},
+                                                             nseg, 0, inner, (const T*)seed, true);
+        if (rc) return rc;
+      }
+      return (int)ACC_SUCCESS;
+    });
+  });
+}

@@ -1,0 +1,87 @@
+// Host-side internals shared by the runtime and the skeleton drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <mutex>
+#include "../../include/accelerate_hip.h"
+
+namespace acc {
+
+struct SyncCtl;
+
+// Sets the thread-local error message and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int hip_fail(hipError_t e, const char* what);
+
+#define ACC_HIP_CHECK(expr)                                  \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return ::acc::hip_fail(_e, #expr); \
+  } while (0)
+
+#define ACC_CHECK(expr)                 \
+  do {                                  \
+    int _s = (expr);                    \
+    if (_s != ACC_SUCCESS) return _s;   \
+  } while (0)
+
+struct DeviceInfo {
+  int compute_units = 0;
+  int max_threads = 0;
+  size_t lds_per_block = 0;
+  size_t total_mem = 0;
+  std::string arch;
+};
+
+int device_info(int device, const DeviceInfo** out);
+
+// Device allocation with the library's accounting (allocation limit knob).
+int dev_malloc(int device, size_t bytes, void** p);
+int dev_free(int device, void* p);
+
+}  // namespace acc
+
+// A stream and the device workspace that its skeletons use.  The workspace is
+// one allocation: a SyncCtl block (tickets, epoch) followed by tile-status
+// words.  It only grows, and growth synchronises the stream first.
+struct acc_stream_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  void* ws = nullptr;          // device: [SyncCtl (256 B) | status words]
+  size_t ws_status_bytes = 0;  // capacity of the status region
+  bool capturing = false;
+};
+
+struct acc_event_s {
+  int device = 0;
+  hipEvent_t event = nullptr;
+};
+
+struct acc_module_s {
+  int device = 0;
+  hipModule_t module = nullptr;
+};
+
+struct acc_graph_s {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace acc {
+
+// Workspace of `s` with at least `status_bytes` of status words; sets the
+// device.  Returns the SyncCtl pointer and the status region.
+int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status);
+
+inline int set_device(int device) {
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return ACC_SUCCESS;
+}
+
+size_t type_size(acc_type t);
+
+}  // namespace acc

@@ -1,0 +1,294 @@
+// permute: forward permutation with an atomic read-modify-write combine.
+//
+// The reference issues one global `atomicrmw` per source element
+// (permute_rmw, PTX/CodeGen/Permute.hs:117-213), so every update of a hot
+// output slot (a Zipf histogram's top bin takes ~16% of them) serialises at
+// the L2 of one channel.  Here, when the output fits in LDS, each workgroup
+// keeps a private copy of the output in LDS, folds its share of the source
+// into it with LDS atomics (for a constant source with (+), lanes of a wave
+// that hit the same slot are first aggregated with a ballot), and flushes the
+// non-neutral slots to the output with one global atomic each.  Larger
+// outputs take one global atomic per element, as the reference does.
+//
+// Combine functions are those the reference turns into atomics
+// (LLVM/CodeGen/Permute.hs:117-145): new (+) old for + min max and or xor,
+// old - new for sub, and new for a combine that ignores the old value
+// (Exchange, :103-106).  Floating min/max use a compare-and-swap loop with
+// fminf/fmaxf semantics, as the reference's CAS emulation does (:187-342).
+// Integer and Bool results are exact (the combines commute); floating (+)
+// depends on the order of atomics, as in the reference.
+//
+// HBM roofline: n*sizeof(I) (+ n*sizeof(T) for a manifest source) + 2*m*sizeof(T).
+#include "dispatch.hpp"
+#include "vecmem.hpp"
+#include "wave64.hpp"
+
+namespace acc {
+
+// ---------------------------------------------------------- atomic combines
+template <typename T> struct AsU;
+template <> struct AsU<float> { using type = uint32_t; };
+template <> struct AsU<double> { using type = uint64_t; };
+
+template <typename T, typename F>
+__device__ __forceinline__ void cas_loop(T* p, T v, F f) {
+  using U = typename AsU<T>::type;
+  U* q = (U*)p;
+  U old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (true) {
+    T o;
+    __builtin_memcpy(&o, &old, sizeof(T));
+    T n = f(v, o);
+    U nu;
+    __builtin_memcpy(&nu, &n, sizeof(T));
+    if (nu == old) return;
+    if (__hip_atomic_compare_exchange_strong(q, &old, nu, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+// Each Rmw has: identity() (neutral for LDS privatisation, when one exists),
+// apply(p, v) at agent scope (global) and lds(p, v) at workgroup scope, and
+// flush(p, v) applying a private partial to the output.
+template <typename T> struct RmwAdd {
+  static constexpr bool privatise = true;
+  static __device__ __forceinline__ T identity() { return T(0); }
+  static __device__ __forceinline__ void apply(T* p, T v) {
+    if constexpr (std::is_floating_point<T>::value) unsafeAtomicAdd(p, v);
+    else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ void lds(T* p, T v) {
+    if constexpr (std::is_floating_point<T>::value) atomicAdd(p, v);
+    else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+};
+
+// old - new: privatised as a sum of the new values, flushed by subtraction
+template <typename T> struct RmwSub {
+  static constexpr bool privatise = true;
+  static __device__ __forceinline__ T identity() { return T(0); }
+  static __device__ __forceinline__ void apply(T* p, T v) {
+    if constexpr (std::is_floating_point<T>::value) unsafeAtomicAdd(p, -v);
+    else __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  static __device__ __forceinline__ void lds(T* p, T v) { RmwAdd<T>::lds(p, v); }
+  static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+};
+
+template <typename T, bool IS_MAX> struct RmwMinMax {
+  static constexpr bool privatise = std::is_integral<T>::value;
+  static __device__ __forceinline__ T identity() {
+    if constexpr (std::is_integral<T>::value)
+      return IS_MAX ? std::numeric_limits<T>::lowest() : std::numeric_limits<T>::max();
+    else
+      return T(0);
+  }
+  static __device__ __forceinline__ void apply(T* p, T v) {
+    if constexpr (std::is_integral<T>::value) {
+      if (IS_MAX) __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      cas_loop(p, v, [](T n, T o) { return IS_MAX ? OpMax{}(n, o) : OpMin{}(n, o); });
+    }
+  }
+  static __device__ __forceinline__ void lds(T* p, T v) {
+    if constexpr (std::is_integral<T>::value) {
+      if (IS_MAX) __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+};
+
+template <typename T, int K> struct RmwBits {  // K: 0 and, 1 or, 2 xor
+  static constexpr bool privatise = true;
+  static __device__ __forceinline__ T identity() { return K == 0 ? (T)~(T)0 : T(0); }
+  template <int SCOPE> static __device__ __forceinline__ void op(T* p, T v) {
+    if (K == 0) __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, SCOPE);
+    else if (K == 1) __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, SCOPE);
+    else __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, SCOPE);
+  }
+  static __device__ __forceinline__ void apply(T* p, T v) { op<__HIP_MEMORY_SCOPE_AGENT>(p, v); }
+  static __device__ __forceinline__ void lds(T* p, T v) { op<__HIP_MEMORY_SCOPE_WORKGROUP>(p, v); }
+  static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+};
+
+template <typename T> struct RmwExchange {
+  static constexpr bool privatise = false;
+  static __device__ __forceinline__ T identity() { return T(0); }
+  static __device__ __forceinline__ void apply(T* p, T v) { *p = v; }
+  static __device__ __forceinline__ void lds(T*, T) {}
+  static __device__ __forceinline__ void flush(T* p, T v) { *p = v; }
+};
+
+template <typename T, typename I> struct PermArgs {
+  const I* idx;
+  const T* src;  // null: constant source
+  T fill;
+  int64_t n;
+  T* out;
+  int64_t m;
+  int64_t head;   // index elements before the first 16-byte aligned one
+  int64_t nvec;   // whole 16-byte index vectors after the head
+  int64_t chunk;  // vectors per workgroup
+};
+
+template <typename I> __device__ __forceinline__ bool in_range(I j, int64_t m) {
+  return (int64_t)j >= 0 && (int64_t)j < m;
+}
+
+// Private LDS copy of the whole output per workgroup.
+template <typename T, typename I, typename R, bool CONST_SRC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_permute_lds(PermArgs<T, I> a) {
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  T* bins = (T*)s_raw;
+  constexpr int VI = 16 / sizeof(I);
+  const T ident = R::identity();
+  for (int64_t b = threadIdx.x; b < a.m; b += BLOCK) bins[b] = ident;
+  __syncthreads();
+
+  auto update = [&](I j, T v) {
+    if (CONST_SRC && std::is_same<R, RmwAdd<T>>::value) {
+      // aggregate lanes of the wave that hit the same slot (hot Zipf bins)
+      const bool ok = in_range(j, a.m);
+      const I lead = (I)__builtin_amdgcn_readfirstlane((int)j);
+      const bool same = ok && j == lead;
+      const uint64_t mask = __ballot(same);
+      if (same) {
+        if (lane_id() == __builtin_ctzll(mask)) R::lds(&bins[j], (T)(__popcll(mask)) * v);
+      } else if (ok) {
+        R::lds(&bins[j], v);
+      }
+    } else {
+      if (in_range(j, a.m)) R::lds(&bins[j], v);
+    }
+  };
+
+  const I* body = a.idx + a.head;
+  const int64_t vb = (int64_t)blockIdx.x * a.chunk;
+  const int64_t ve = min(vb + a.chunk, a.nvec);
+  for (int64_t i = vb + threadIdx.x; i < ve; i += BLOCK) {
+    Vec<I> iv = load16_stream(body + i * VI);
+#pragma unroll
+    for (int k = 0; k < VI; ++k) {
+      const int64_t e = a.head + i * VI + k;
+      update(iv.e[k], CONST_SRC ? a.fill : a.src[e]);
+    }
+  }
+  // head and tail elements (outside whole vectors): the last workgroup
+  if (blockIdx.x == gridDim.x - 1) {
+    const int64_t tail0 = a.head + a.nvec * VI;
+    for (int64_t e = threadIdx.x; e < a.head; e += BLOCK) {
+      const I j = a.idx[e];
+      if (in_range(j, a.m)) R::lds(&bins[j], CONST_SRC ? a.fill : a.src[e]);
+    }
+    for (int64_t e = tail0 + threadIdx.x; e < a.n; e += BLOCK) {
+      const I j = a.idx[e];
+      if (in_range(j, a.m)) R::lds(&bins[j], CONST_SRC ? a.fill : a.src[e]);
+    }
+  }
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < a.m; b += BLOCK) {
+    const T v = bins[b];
+    if (v != ident) R::flush(&a.out[b], v);
+  }
+}
+
+// One global atomic per element (large outputs, Exchange, floating min/max).
+template <typename T, typename I, typename R, bool CONST_SRC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_permute_direct(PermArgs<T, I> a) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < a.n; e += stride) {
+    const I j = a.idx[e];
+    if (in_range(j, a.m)) R::apply(&a.out[j], CONST_SRC ? a.fill : a.src[e]);
+  }
+}
+
+template <typename T, typename I, typename R>
+int permute_launch(acc_stream_t s, T* out, int64_t m, const T* src, T fill, const I* idx, int64_t n) {
+  constexpr int BLOCK = 512;
+  constexpr int VI = 16 / sizeof(I);
+  const DeviceInfo* di;
+  ACC_CHECK(device_info(s->device, &di));
+  if ((uintptr_t)idx % sizeof(I)) return fail(ACC_ERROR_INVALID_VALUE, "permute: index not element-aligned");
+  int64_t head = (int64_t)(((16 - (uintptr_t)idx % 16) % 16) / sizeof(I));
+  if (head > n) head = n;
+  const int64_t nvec = (n - head) / VI;
+  PermArgs<T, I> a{idx, src, fill, n, out, m, head, nvec, 0};
+  const size_t lds = (size_t)m * sizeof(T);
+  const bool privatise = R::privatise && lds <= 64 * 1024;
+  if (privatise) {
+    int64_t nb = ceil_div(nvec, (int64_t)BLOCK * 8);
+    const int64_t maxb = (int64_t)di->compute_units * 2;
+    if (nb > maxb) nb = maxb;
+    if (nb < 1) nb = 1;
+    a.chunk = nvec ? ceil_div(nvec, nb) : 0;
+    if (a.chunk) nb = max((int64_t)1, ceil_div(nvec, a.chunk));
+    if (src) hipLaunchKernelGGL((k_permute_lds<T, I, R, false, BLOCK>), dim3((unsigned)nb), dim3(BLOCK), lds, s->stream, a);
+    else hipLaunchKernelGGL((k_permute_lds<T, I, R, true, BLOCK>), dim3((unsigned)nb), dim3(BLOCK), lds, s->stream, a);
+  } else {
+    int64_t nb = ceil_div(n, (int64_t)BLOCK);
+    const int64_t maxb = (int64_t)di->compute_units * 8;
+    if (nb > maxb) nb = maxb;
+    if (src) hipLaunchKernelGGL((k_permute_direct<T, I, R, false, BLOCK>), dim3((unsigned)nb), dim3(BLOCK), 0, s->stream, a);
+    else hipLaunchKernelGGL((k_permute_direct<T, I, R, true, BLOCK>), dim3((unsigned)nb), dim3(BLOCK), 0, s->stream, a);
+  }
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
+
+template <typename T, typename F>
+int with_rmw(acc_op op, F&& f) {
+  switch (op) {
+    case ACC_OP_ADD: return f(RmwAdd<T>{});
+    case ACC_OP_SUB: return f(RmwSub<T>{});
+    case ACC_OP_MIN: return f(RmwMinMax<T, false>{});
+    case ACC_OP_MAX: return f(RmwMinMax<T, true>{});
+    case ACC_OP_EXCHANGE: return f(RmwExchange<T>{});
+    default: break;
+  }
+  if constexpr (std::is_integral<T>::value) {
+    switch (op) {
+      case ACC_OP_AND: return f(RmwBits<T, 0>{});
+      case ACC_OP_OR: return f(RmwBits<T, 1>{});
+      case ACC_OP_XOR: return f(RmwBits<T, 2>{});
+      default: break;
+    }
+  }
+  return fail(ACC_ERROR_UNSUPPORTED, "permute: combine function %d for this element type", (int)op);
+}
+
+}  // namespace acc
+
+using namespace acc;
+
+extern "C" int acc_hip_permute(acc_stream_t s, acc_type t, acc_op combine, const void* defaults, void* out,
+                               int64_t m, const void* src, const void* src_fill, acc_type index_type,
+                               const void* index, int64_t n) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "permute: null stream");
+  if (m < 0 || n < 0) return fail(ACC_ERROR_INVALID_VALUE, "permute: negative extent");
+  if (index_type != ACC_INT32 && index_type != ACC_INT64)
+    return fail(ACC_ERROR_UNSUPPORTED, "permute: index type must be Int32 or Int64");
+  const size_t B = type_size(t);
+  if (!B) return fail(ACC_ERROR_UNSUPPORTED, "permute: element type %d", (int)t);
+  if (m > 0 && (!out || !defaults)) return fail(ACC_ERROR_INVALID_VALUE, "permute: null output");
+  if (n > 0 && !index) return fail(ACC_ERROR_INVALID_VALUE, "permute: null index");
+  if (n > 0 && !src && !src_fill) return fail(ACC_ERROR_INVALID_VALUE, "permute: no source");
+  ACC_CHECK(set_device(s->device));
+  // clone the defaults unless updating in place (PTX/Execute.hs:623-625)
+  if (out != defaults && m > 0)
+    ACC_HIP_CHECK(hipMemcpyAsync(out, defaults, (size_t)m * B, hipMemcpyDeviceToDevice, s->stream));
+  if (n == 0 || m == 0) return ACC_SUCCESS;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    const T fill = src_fill ? *(const T*)src_fill : T();
+    return with_rmw<T>(combine, [&](auto r) {
+      using R = decltype(r);
+      if (index_type == ACC_INT32)
+        return permute_launch<T, int32_t, R>(s, (T*)out, m, (const T*)src, fill, (const int32_t*)index, n);
+      return permute_launch<T, int64_t, R>(s, (T*)out, m, (const T*)src, fill, (const int64_t*)index, n);
+    });
+  });
+}

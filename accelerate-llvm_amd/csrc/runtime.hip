@@ -1,0 +1,484 @@
+// Runtime half of the C ABI: devices, memory, transfers, streams, events,
+// code objects, launches and graphs.  Each function replaces one group of
+// `Foreign.CUDA.*` call sites of accelerate-llvm-ptx (cited in
+// include/accelerate_hip.h).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <unordered_map>
+#include <vector>
+#include <memory>
+#include "host.hpp"
+#include "lookback.hpp"
+
+namespace acc {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return fail(ACC_ERROR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+  if (e == hipErrorNoDevice) return fail(ACC_ERROR_NO_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  return fail(ACC_ERROR_HIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+size_t type_size(acc_type t) {
+  switch (t) {
+    case ACC_INT32: case ACC_WORD32: case ACC_FLOAT32: return 4;
+    case ACC_INT64: case ACC_WORD64: case ACC_FLOAT64: return 8;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ devices
+static std::mutex g_dev_mu;
+static std::vector<std::unique_ptr<DeviceInfo>> g_devs;
+
+int device_info(int device, const DeviceInfo** out) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  if (g_devs.empty()) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    g_devs.resize(n);
+  }
+  if (device < 0 || device >= (int)g_devs.size())
+    return fail(ACC_ERROR_INVALID_VALUE, "device %d out of range (%zu devices)", device, g_devs.size());
+  if (!g_devs[device]) {
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDeviceProperties(&p, device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    auto d = std::make_unique<DeviceInfo>();
+    d->compute_units = p.multiProcessorCount;
+    d->max_threads = p.maxThreadsPerBlock;
+    d->lds_per_block = p.sharedMemPerBlock;
+    d->total_mem = p.totalGlobalMem;
+    d->arch = p.gcnArchName;
+    g_devs[device] = std::move(d);
+  }
+  *out = g_devs[device].get();
+  return ACC_SUCCESS;
+}
+
+// ------------------------------------------------------------------- memory
+struct Accounting {
+  std::mutex mu;
+  std::unordered_map<void*, size_t> live;
+  size_t bytes = 0;
+  size_t limit = 0;
+};
+static Accounting g_acct[64];
+
+int dev_malloc(int device, size_t bytes, void** p) {
+  if (device < 0 || device >= 64) return fail(ACC_ERROR_INVALID_VALUE, "bad device %d", device);
+  Accounting& a = g_acct[device];
+  {
+    std::lock_guard<std::mutex> lk(a.mu);
+    if (a.limit && a.bytes + bytes > a.limit)
+      return fail(ACC_ERROR_OUT_OF_MEMORY, "allocation of %zu bytes exceeds limit (%zu of %zu in use)",
+                  bytes, a.bytes, a.limit);
+  }
+  ACC_CHECK(set_device(device));
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return hip_fail(e, "hipMalloc");
+  }
+  std::lock_guard<std::mutex> lk(a.mu);
+  a.live[q] = bytes;
+  a.bytes += bytes;
+  *p = q;
+  return ACC_SUCCESS;
+}
+
+int dev_free(int device, void* p) {
+  if (!p) return ACC_SUCCESS;
+  if (device < 0 || device >= 64) return fail(ACC_ERROR_INVALID_VALUE, "bad device %d", device);
+  Accounting& a = g_acct[device];
+  {
+    std::lock_guard<std::mutex> lk(a.mu);
+    auto it = a.live.find(p);
+    if (it == a.live.end()) return fail(ACC_ERROR_INVALID_VALUE, "free of unknown pointer %p", p);
+    a.bytes -= it->second;
+    a.live.erase(it);
+  }
+  ACC_CHECK(set_device(device));
+  ACC_HIP_CHECK(hipFree(p));
+  return ACC_SUCCESS;
+}
+
+// ------------------------------------------------------------------ streams
+static constexpr size_t kCtlBytes = 256;
+
+int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
+  ACC_CHECK(set_device(s->device));
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (!s->ws || status_bytes > s->ws_status_bytes) {
+    if (s->capturing)
+      return fail(ACC_ERROR_INVALID_VALUE,
+                  "stream workspace must grow to %zu bytes during capture; call "
+                  "acc_hip_stream_reserve before capturing", status_bytes);
+    size_t cap = s->ws_status_bytes ? s->ws_status_bytes : (size_t)1 << 20;
+    while (cap < status_bytes) cap *= 2;
+    void* ws = nullptr;
+    ACC_CHECK(dev_malloc(s->device, kCtlBytes + cap, &ws));
+    ACC_HIP_CHECK(hipMemsetAsync(ws, 0, kCtlBytes + cap, s->stream));
+    if (s->ws) {
+      // the old workspace may still be in use by queued work
+      ACC_HIP_CHECK(hipStreamSynchronize(s->stream));
+      ACC_CHECK(dev_free(s->device, s->ws));
+    }
+    s->ws = ws;
+    s->ws_status_bytes = cap;
+  }
+  *ctl = (SyncCtl*)s->ws;
+  *status = (uint64_t*)((char*)s->ws + kCtlBytes);
+  return ACC_SUCCESS;
+}
+
+// ---------------------------------------------------------------- fill kernel
+template <typename W>
+__global__ void k_fill(W* __restrict__ dst, W value, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) dst[i] = value;
+}
+
+}  // namespace acc
+
+using namespace acc;
+
+extern "C" {
+
+const char* acc_hip_last_error(void) { return g_last_error.c_str(); }
+int acc_hip_abi_version(void) { return ACC_HIP_ABI_VERSION; }
+
+int acc_hip_device_count(int* count) {
+  if (!count) return fail(ACC_ERROR_INVALID_VALUE, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e == hipErrorNoDevice) { *count = 0; (void)hipGetLastError(); return ACC_SUCCESS; }
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+  *count = n;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_device_props(int device, acc_device_props* props) {
+  if (!props) return fail(ACC_ERROR_INVALID_VALUE, "null props");
+  hipDeviceProp_t p;
+  ACC_HIP_CHECK(hipGetDeviceProperties(&p, device));
+  memset(props, 0, sizeof *props);
+  snprintf(props->name, sizeof props->name, "%s", p.name);
+  snprintf(props->arch, sizeof props->arch, "%s", p.gcnArchName);
+  props->compute_units = p.multiProcessorCount;
+  props->wave_size = p.warpSize;
+  props->max_threads_per_block = p.maxThreadsPerBlock;
+  props->lds_per_block = p.sharedMemPerBlock;
+  props->total_global_mem = p.totalGlobalMem;
+  props->clock_khz = p.clockRate;
+  props->memory_clock_khz = p.memoryClockRate;
+  props->memory_bus_width = p.memoryBusWidth;
+  props->l2_cache_bytes = p.l2CacheSize;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_device_synchronize(int device) {
+  ACC_CHECK(set_device(device));
+  ACC_HIP_CHECK(hipDeviceSynchronize());
+  return ACC_SUCCESS;
+}
+
+int acc_hip_malloc(int device, size_t bytes, void** ptr) {
+  if (!ptr) return fail(ACC_ERROR_INVALID_VALUE, "null ptr");
+  return dev_malloc(device, bytes, ptr);
+}
+
+int acc_hip_free(int device, void* ptr) { return dev_free(device, ptr); }
+
+int acc_hip_mem_get_info(int device, size_t* free_bytes, size_t* total_bytes) {
+  ACC_CHECK(set_device(device));
+  size_t f = 0, t = 0;
+  ACC_HIP_CHECK(hipMemGetInfo(&f, &t));
+  if (device >= 0 && device < 64) {
+    Accounting& a = g_acct[device];
+    std::lock_guard<std::mutex> lk(a.mu);
+    if (a.limit) {
+      size_t left = a.limit > a.bytes ? a.limit - a.bytes : 0;
+      if (left < f) f = left;
+      if (a.limit < t) t = a.limit;
+    }
+  }
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_set_alloc_limit(int device, size_t limit_bytes) {
+  if (device < 0 || device >= 64) return fail(ACC_ERROR_INVALID_VALUE, "bad device %d", device);
+  std::lock_guard<std::mutex> lk(g_acct[device].mu);
+  g_acct[device].limit = limit_bytes;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_allocated_bytes(int device, size_t* bytes) {
+  if (device < 0 || device >= 64 || !bytes) return fail(ACC_ERROR_INVALID_VALUE, "bad argument");
+  std::lock_guard<std::mutex> lk(g_acct[device].mu);
+  *bytes = g_acct[device].bytes;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_host_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(ACC_ERROR_INVALID_VALUE, "null ptr");
+  ACC_HIP_CHECK(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_host_free(void* ptr) {
+  if (!ptr) return ACC_SUCCESS;
+  ACC_HIP_CHECK(hipHostFree(ptr));
+  return ACC_SUCCESS;
+}
+
+static int copy_async(acc_stream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind k) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
+  if (!bytes) return ACC_SUCCESS;
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, s->stream));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_memcpy_h2d_async(acc_stream_t s, void* dst, const void* src, size_t bytes) {
+  return copy_async(s, dst, src, bytes, hipMemcpyHostToDevice);
+}
+int acc_hip_memcpy_d2h_async(acc_stream_t s, void* dst, const void* src, size_t bytes) {
+  return copy_async(s, dst, src, bytes, hipMemcpyDeviceToHost);
+}
+int acc_hip_memcpy_d2d_async(acc_stream_t s, void* dst, const void* src, size_t bytes) {
+  return copy_async(s, dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+int acc_hip_fill_async(acc_stream_t s, void* dst, const void* value, size_t elem_bytes, size_t count) {
+  if (!s || !value) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  if (!count) return ACC_SUCCESS;
+  if (!dst) return fail(ACC_ERROR_INVALID_VALUE, "null destination");
+  ACC_CHECK(set_device(s->device));
+  unsigned grid = (unsigned)((count + 255) / 256);
+  if (grid > 8192) grid = 8192;
+  switch (elem_bytes) {
+    case 1: { uint8_t v; memcpy(&v, value, 1); hipLaunchKernelGGL(k_fill<uint8_t>, grid, 256, 0, s->stream, (uint8_t*)dst, v, count); break; }
+    case 2: { uint16_t v; memcpy(&v, value, 2); hipLaunchKernelGGL(k_fill<uint16_t>, grid, 256, 0, s->stream, (uint16_t*)dst, v, count); break; }
+    case 4: { uint32_t v; memcpy(&v, value, 4); hipLaunchKernelGGL(k_fill<uint32_t>, grid, 256, 0, s->stream, (uint32_t*)dst, v, count); break; }
+    case 8: { uint64_t v; memcpy(&v, value, 8); hipLaunchKernelGGL(k_fill<uint64_t>, grid, 256, 0, s->stream, (uint64_t*)dst, v, count); break; }
+    default: return fail(ACC_ERROR_INVALID_VALUE, "fill element size %zu", elem_bytes);
+  }
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
+
+int acc_hip_stream_create(int device, acc_stream_t* out) {
+  if (!out) return fail(ACC_ERROR_INVALID_VALUE, "null stream out");
+  const DeviceInfo* di;
+  ACC_CHECK(device_info(device, &di));
+  ACC_CHECK(set_device(device));
+  auto s = std::make_unique<acc_stream_s>();
+  s->device = device;
+  ACC_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  *out = s.release();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_stream_destroy(acc_stream_t s) {
+  if (!s) return ACC_SUCCESS;
+  int rc = set_device(s->device);
+  if (rc == ACC_SUCCESS) {
+    (void)hipStreamSynchronize(s->stream);
+    if (s->ws) dev_free(s->device, s->ws);
+    (void)hipStreamDestroy(s->stream);
+  }
+  delete s;
+  return rc;
+}
+
+int acc_hip_stream_synchronize(acc_stream_t s) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipStreamSynchronize(s->stream));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_stream_query(acc_stream_t s) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
+  hipError_t e = hipStreamQuery(s->stream);
+  if (e == hipSuccess) return ACC_SUCCESS;
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); return ACC_ERROR_NOT_READY; }
+  return hip_fail(e, "hipStreamQuery");
+}
+
+int acc_hip_stream_device(acc_stream_t s, int* device) {
+  if (!s || !device) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  *device = s->device;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_stream_reserve(acc_stream_t s, size_t status_bytes) {
+  SyncCtl* c;
+  uint64_t* st;
+  return stream_workspace(s, status_bytes, &c, &st);
+}
+
+int acc_hip_event_create(int device, int timing, acc_event_t* out) {
+  if (!out) return fail(ACC_ERROR_INVALID_VALUE, "null event out");
+  ACC_CHECK(set_device(device));
+  auto e = std::make_unique<acc_event_s>();
+  e->device = device;
+  ACC_HIP_CHECK(hipEventCreateWithFlags(&e->event, timing ? hipEventDefault : hipEventDisableTiming));
+  *out = e.release();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_event_destroy(acc_event_t e) {
+  if (!e) return ACC_SUCCESS;
+  (void)hipEventDestroy(e->event);
+  delete e;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_event_record(acc_event_t e, acc_stream_t s) {
+  if (!e || !s) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipEventRecord(e->event, s->stream));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_stream_wait_event(acc_stream_t s, acc_event_t e) {
+  if (!e || !s) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipStreamWaitEvent(s->stream, e->event, 0));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_event_query(acc_event_t e) {
+  if (!e) return fail(ACC_ERROR_INVALID_VALUE, "null event");
+  hipError_t r = hipEventQuery(e->event);
+  if (r == hipSuccess) return ACC_SUCCESS;
+  if (r == hipErrorNotReady) { (void)hipGetLastError(); return ACC_ERROR_NOT_READY; }
+  return hip_fail(r, "hipEventQuery");
+}
+
+int acc_hip_event_synchronize(acc_event_t e) {
+  if (!e) return fail(ACC_ERROR_INVALID_VALUE, "null event");
+  ACC_HIP_CHECK(hipEventSynchronize(e->event));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_event_elapsed_ms(acc_event_t a, acc_event_t b, float* ms) {
+  if (!a || !b || !ms) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  ACC_HIP_CHECK(hipEventElapsedTime(ms, a->event, b->event));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_module_load_data(int device, const void* image, size_t bytes, acc_module_t* out) {
+  (void)bytes;
+  if (!image || !out) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  ACC_CHECK(set_device(device));
+  auto m = std::make_unique<acc_module_s>();
+  m->device = device;
+  ACC_HIP_CHECK(hipModuleLoadData(&m->module, image));
+  *out = m.release();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_module_unload(acc_module_t m) {
+  if (!m) return ACC_SUCCESS;
+  (void)set_device(m->device);
+  hipError_t e = hipModuleUnload(m->module);
+  delete m;
+  if (e != hipSuccess) return hip_fail(e, "hipModuleUnload");
+  return ACC_SUCCESS;
+}
+
+int acc_hip_module_get_function(acc_module_t m, const char* name, acc_function_t* f) {
+  if (!m || !name || !f) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  hipFunction_t fn;
+  ACC_HIP_CHECK(hipModuleGetFunction(&fn, m->module, name));
+  *f = (acc_function_t)fn;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_function_attributes(acc_function_t f, int* regs, int* lds, int* max_threads) {
+  if (!f) return fail(ACC_ERROR_INVALID_VALUE, "null function");
+  hipFunction_t fn = (hipFunction_t)f;
+  int v = 0;
+  if (regs) { ACC_HIP_CHECK(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn)); *regs = v; }
+  if (lds) { ACC_HIP_CHECK(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fn)); *lds = v; }
+  if (max_threads) { ACC_HIP_CHECK(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, fn)); *max_threads = v; }
+  return ACC_SUCCESS;
+}
+
+int acc_hip_launch(acc_function_t f, unsigned gx, unsigned gy, unsigned gz, unsigned bx, unsigned by,
+                   unsigned bz, unsigned lds, acc_stream_t s, void** args) {
+  if (!f || !s) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  if ((uint64_t)gx * gy * gz == 0) return ACC_SUCCESS;  // zero-size launches are skipped (PTX/Execute.hs:847-854)
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipModuleLaunchKernel((hipFunction_t)f, gx, gy, gz, bx, by, bz, lds, s->stream, args, nullptr));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_graph_begin_capture(acc_stream_t s) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->capturing = true;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_graph_end_capture(acc_stream_t s, acc_graph_t* out) {
+  if (!s || !out) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  ACC_CHECK(set_device(s->device));
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->capturing = false;
+  }
+  auto g = std::make_unique<acc_graph_s>();
+  g->device = s->device;
+  ACC_HIP_CHECK(hipStreamEndCapture(s->stream, &g->graph));
+  hipError_t e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *out = g.release();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_graph_launch(acc_graph_t g, acc_stream_t s) {
+  if (!g || !s) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  ACC_CHECK(set_device(s->device));
+  ACC_HIP_CHECK(hipGraphLaunch(g->exec, s->stream));
+  return ACC_SUCCESS;
+}
+
+int acc_hip_graph_destroy(acc_graph_t g) {
+  if (!g) return ACC_SUCCESS;
+  (void)set_device(g->device);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return ACC_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,317 @@
+// scanl/scanl1/scanl'/scanr/scanr1/scanr' over a vector in ONE pass.
+//
+// The reference runs three kernels (scanP1 block scans, scanP2 scan of block
+// totals, scanP3 carry add; PTX/Execute.hs:422-468, PTX/CodeGen/Scan.hs:141-439)
+// and moves ~4n elements.  This kernel is a single-pass decoupled look-back
+// scan: each workgroup takes a tile by ticket, scans it in registers (thread
+// scan of dwordx4 vectors, DPP wave scan, LDS wave-total scan), publishes the
+// tile aggregate, looks back over predecessor tiles with one wave, and writes
+// its outputs: n reads + n writes.
+//
+// Element semantics (NAT/CodeGen/Scan.hs:136-270):
+//   left-to-right combines `acc (+) x`, right-to-left `x (+) acc`; a
+//   right-to-left scan is the left-to-right scan of the reversed sequence with
+//   the arguments flipped, so both directions share this kernel.  In traversal
+//   order with prefix seed z (when given):
+//     scanl1/scanr1 : out[i] = inclusive value at i
+//     scanl z       : out[i] = exclusive value at i, out[n] = total
+//     scanr z       : out[i] = inclusive value at i, out[n] = z
+//     scanl'/scanr' : out[i] = exclusive value at i, sums = total
+//   where "at i" is the element's own position in the array.
+#include "dispatch.hpp"
+#include "lookback.hpp"
+#include "vecmem.hpp"
+
+namespace acc {
+
+template <typename T> struct ScanArgs {
+  const T* in;
+  T* out;
+  int64_t n;
+  uint32_t ntiles;
+  uint32_t exclusive;  // emit exclusive values
+  uint32_t has_seed;
+  T seed;
+  T* total_out;  // nullable: where the total goes
+  T* seed_out;   // nullable: where the seed goes (scanr z: out[n])
+  SyncCtl* ctl;
+  uint64_t* status;
+};
+
+template <typename T, typename Op, bool MASKED> struct Lane;
+
+// Element storage without masks: plain values.
+template <typename T, typename Op> struct Lane<T, Op, false> {
+  using E = T;
+  static __device__ __forceinline__ E make(T v, bool) { return v; }
+  static __device__ __forceinline__ E cmb(Op op, E a, E b) { return op(a, b); }
+  static __device__ __forceinline__ Maybe<T> lift(E e) { return Maybe<T>{e, 1u}; }
+  static __device__ __forceinline__ E wscan(E x, Op op) { return wave_inclusive_scan(x, op); }
+};
+
+// Element storage in the ragged tile: optional values.
+template <typename T, typename Op> struct Lane<T, Op, true> {
+  using E = Maybe<T>;
+  static __device__ __forceinline__ E make(T v, bool ok) { return Maybe<T>{v, ok ? 1u : 0u}; }
+  static __device__ __forceinline__ E cmb(Op op, E a, E b) { return mcombine(op, a, b); }
+  static __device__ __forceinline__ Maybe<T> lift(E e) { return e; }
+  static __device__ __forceinline__ E wscan(E x, Op op) { return wave_inclusive_scan_m(x, op); }
+};
+
+template <typename T> __device__ __forceinline__ Maybe<T> none() {
+  Maybe<T> m;
+  m.v = T();
+  m.ok = 0;
+  return m;
+}
+
+template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED, bool MASKED>
+__device__ __forceinline__ void scan_tile(const ScanArgs<T>& a, Op op, uint32_t t, int64_t base,
+                                          uint32_t epoch, Maybe<T>* s_wave, Maybe<T>* s_prefix) {
+  constexpr int VEC = Vec<T>::N;
+  constexpr int W = BLOCK / 64;
+  constexpr int NW = Tagged<T>::NW;
+  using L = Lane<T, Op, MASKED>;
+  using E = typename L::E;
+  const int lane = lane_id();
+  const int w = threadIdx.x / 64;
+
+  // physical vector index (within the tile) of logical chunk k of this lane
+  auto vidx = [&](int k) -> int64_t {
+    return REVERSE ? ((int64_t)((W - 1 - w) * K + (K - 1 - k)) * 64 + (63 - lane))
+                   : ((int64_t)(w * K + k) * 64 + lane);
+  };
+
+  // ---- load, thread-level inclusive scan of each vector
+  E x[K][VEC];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t e0 = base + vidx(k) * VEC;
+    T v[VEC];
+    bool ok[VEC];
+    if (ALIGNED && !MASKED) {
+      Vec<T> r = load16_stream(a.in + e0);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) { v[j] = r.e[j]; ok[j] = true; }
+    } else {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const int64_t i = e0 + j;
+        ok[j] = !MASKED || i < a.n;
+        v[j] = ok[j] ? a.in[i] : T();
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const int jj = REVERSE ? VEC - 1 - j : j;
+      x[k][j] = L::make(v[jj], ok[jj]);
+    }
+#pragma unroll
+    for (int j = 1; j < VEC; ++j) x[k][j] = L::cmb(op, x[k][j - 1], x[k][j]);
+  }
+
+  // ---- wave scan of lane totals, chunk by chunk
+  Maybe<T> pre[K];
+  Maybe<T> run = none<T>();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    E inc = L::wscan(x[k][VEC - 1], op);
+    E up = shift_up1(inc);
+    Maybe<T> exl = lane > 0 ? L::lift(up) : none<T>();
+    pre[k] = mcombine(op, run, exl);
+    run = mcombine(op, run, L::lift(read_lane(inc, 63)));
+  }
+  if (lane == 0) s_wave[w] = run;
+  __syncthreads();
+  Maybe<T> wp = none<T>(), agg = none<T>();
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const Maybe<T> v = s_wave[i];
+    if (i < w) wp = mcombine(op, wp, v);
+    agg = mcombine(op, agg, v);
+  }
+
+  // ---- tile prefix: seed for the first tile, look-back otherwise
+  if (w == 0) {
+    Maybe<T> X = none<T>();
+    uint64_t* slot = a.status + (size_t)t * NW;
+    if (t == 0) {
+      if (a.has_seed) X = Maybe<T>{a.seed, 1u};
+    } else {
+      if (lane == 0) Tagged<T>::publish(slot, agg.v, make_tag(epoch, ST_AGGREGATE));
+      X = Maybe<T>{wave_lookback<T>(a.status, t, epoch, op), 1u};
+    }
+    const Maybe<T> incl = mcombine(op, X, agg);
+    if (lane == 0) {
+      if (t + 1 < a.ntiles) Tagged<T>::publish(slot, incl.v, make_tag(epoch, ST_PREFIX));
+      *s_prefix = X;
+      if (t + 1 == a.ntiles && a.total_out) *a.total_out = incl.v;
+      if (t == 0 && a.seed_out) *a.seed_out = a.seed;
+    }
+  }
+  __syncthreads();
+  const Maybe<T> X = *s_prefix;
+  const Maybe<T> XW = mcombine(op, X, wp);
+
+  // ---- outputs
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const Maybe<T> P = mcombine(op, XW, pre[k]);
+    T y[VEC];
+    bool ok[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const Maybe<T> xi = L::lift(x[k][j]);
+      if (a.exclusive) {
+        // exclusive value: everything before this element (always defined: a seed is given)
+        const Maybe<T> prev = j == 0 ? P : mcombine(op, P, L::lift(x[k][j - 1]));
+        y[j] = prev.v;
+      } else {
+        y[j] = mcombine(op, P, xi).v;
+      }
+      ok[j] = true;
+    }
+    if constexpr (MASKED) {
+      // validity of each element in its own (physical) position
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const int64_t i = base + vidx(k) * VEC + (REVERSE ? VEC - 1 - j : j);
+        ok[j] = i < a.n;
+      }
+    }
+    const int64_t e0 = base + vidx(k) * VEC;
+    if (ALIGNED && !MASKED) {
+      Vec<T> r;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) r.e[j] = y[REVERSE ? VEC - 1 - j : j];
+      store16(a.out + e0, r);
+    } else {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const int jl = REVERSE ? VEC - 1 - j : j;  // logical slot of physical element j
+        if (ok[jl]) a.out[e0 + j] = y[jl];
+      }
+    }
+  }
+}
+
+template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED>
+__global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs<T> a) {
+  constexpr int VEC = Vec<T>::N;
+  constexpr int TILE = BLOCK * K * VEC;
+  __shared__ uint32_t s_u32[2];
+  __shared__ Maybe<T> s_wave[BLOCK / 64];
+  __shared__ Maybe<T> s_prefix;
+  using OpT = typename std::conditional<REVERSE, Flip<Op>, Op>::type;
+  OpT op{};
+  const uint32_t epoch = block_epoch(a.ctl, &s_u32[0]);
+  const uint32_t t = block_ticket(a.ctl, &s_u32[1]);
+  const int64_t p = REVERSE ? (int64_t)(a.ntiles - 1 - t) : (int64_t)t;
+  const int64_t base = p * TILE;
+  if (base + TILE <= a.n)
+    scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, false>(a, op, t, base, epoch, s_wave, &s_prefix);
+  else
+    scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, true>(a, op, t, base, epoch, s_wave, &s_prefix);
+  block_retire(a.ctl, gridDim.x);
+}
+
+template <typename T> struct ScanTune {
+  static constexpr int BLOCK = 256;
+  static constexpr int K = 4;  // dwordx4 vectors per thread
+};
+
+template <typename T, typename Op>
+int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const T* in, int64_t n,
+             T* out, T* total_out, T* seed_out) {
+  constexpr int BLOCK = ScanTune<T>::BLOCK, K = ScanTune<T>::K, VEC = Vec<T>::N;
+  constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
+  constexpr int NW = Tagged<T>::NW;
+  const int64_t ntiles = ceil_div(n, TILE);
+  if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "scan: %lld elements", (long long)n);
+  if ((uintptr_t)in % sizeof(T) || (uintptr_t)out % sizeof(T))
+    return fail(ACC_ERROR_INVALID_VALUE, "scan: arrays not element-aligned");
+  const bool aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+  SyncCtl* ctl;
+  uint64_t* status;
+  ACC_CHECK(stream_workspace(s, (size_t)ntiles * NW * 8, &ctl, &status));
+  ScanArgs<T> a{in, out, n, (uint32_t)ntiles, exclusive ? 1u : 0u, seed ? 1u : 0u, seed ? *seed : T(),
+                total_out, seed_out, ctl, status};
+  dim3 g((unsigned)ntiles), b(BLOCK);
+  if (reverse) {
+    if (aligned) hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, true, true>), g, b, 0, s->stream, a);
+    else hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, true, false>), g, b, 0, s->stream, a);
+  } else {
+    if (aligned) hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, false, true>), g, b, 0, s->stream, a);
+    else hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, false, false>), g, b, 0, s->stream, a);
+  }
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
+
+// rank >= 2 scans: one call per row for now (each row is an independent scan)
+template <typename T, typename Op>
+int scan_rows(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const T* in,
+              int64_t outer, int64_t inner, T* out, int64_t out_row, T* sums, bool seed_at_end) {
+  for (int64_t r = 0; r < outer; ++r) {
+    T* o = out + r * out_row;
+    T* total = sums ? sums + r : (exclusive && !reverse && !sums ? o + inner : nullptr);
+    T* seed_out = seed_at_end ? o + inner : nullptr;
+    ACC_CHECK((scan_all<T, Op>(s, reverse, exclusive, seed, in + r * inner, inner, o, total, seed_out)));
+  }
+  return ACC_SUCCESS;
+}
+
+}  // namespace acc
+
+using namespace acc;
+
+extern "C" int acc_hip_scan(acc_stream_t s, acc_type t, acc_op op, acc_direction dir, const void* seed,
+                            const void* in, int64_t outer, int64_t inner, void* out) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "scan: null stream");
+  if (outer < 0 || inner < 0) return fail(ACC_ERROR_INVALID_VALUE, "scan: negative extent");
+  const size_t B = type_size(t);
+  if (!B) return fail(ACC_ERROR_UNSUPPORTED, "scan: element type %d", (int)t);
+  if (outer == 0) return ACC_SUCCESS;
+  if (inner == 0) {
+    // scanl1 of an empty row is empty; scanl z gives [z] per row (PTX/Execute.hs:386-389)
+    if (!seed) return ACC_SUCCESS;
+    return acc_hip_fill_async(s, out, seed, B, (size_t)outer);
+  }
+  if (!in || !out) return fail(ACC_ERROR_INVALID_VALUE, "scan: null array");
+  ACC_CHECK(set_device(s->device));
+  const bool reverse = dir == ACC_RIGHT_TO_LEFT;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    return with_op<T>(op, [&](auto f) {
+      using Op = decltype(f);
+      const T* sd = (const T*)seed;
+      if (!sd) return scan_rows<T, Op>(s, reverse, false, nullptr, (const T*)in, outer, inner, (T*)out, inner, nullptr, false);
+      // scanl z: exclusive values + total at out[n]; scanr z: inclusive values + z at out[n]
+      return scan_rows<T, Op>(s, reverse, !reverse, sd, (const T*)in, outer, inner, (T*)out, inner + 1,
+                              nullptr, reverse);
+    });
+  });
+}
+
+extern "C" int acc_hip_scan_prime(acc_stream_t s, acc_type t, acc_op op, acc_direction dir, const void* seed,
+                                  const void* in, int64_t outer, int64_t inner, void* out, void* sums) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "scan': null stream");
+  if (!seed) return fail(ACC_ERROR_INVALID_VALUE, "scan': seed required");
+  if (outer < 0 || inner < 0) return fail(ACC_ERROR_INVALID_VALUE, "scan': negative extent");
+  const size_t B = type_size(t);
+  if (!B) return fail(ACC_ERROR_UNSUPPORTED, "scan': element type %d", (int)t);
+  if (outer == 0) return ACC_SUCCESS;
+  // empty rows: no prefixes, the sums are the seed (PTX/Execute.hs:500-509)
+  if (inner == 0) return acc_hip_fill_async(s, sums, seed, B, (size_t)outer);
+  if (!in || !out || !sums) return fail(ACC_ERROR_INVALID_VALUE, "scan': null array");
+  ACC_CHECK(set_device(s->device));
+  const bool reverse = dir == ACC_RIGHT_TO_LEFT;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    return with_op<T>(op, [&](auto f) {
+      using Op = decltype(f);
+      return scan_rows<T, Op>(s, reverse, true, (const T*)seed, (const T*)in, outer, inner, (T*)out, inner,
+                              (T*)sums, false);
+    });
+  });
+}

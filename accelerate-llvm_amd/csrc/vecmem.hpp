@@ -1,0 +1,42 @@
+// 16-byte (dwordx4) global loads and stores of VEC = 16 / sizeof(T) elements.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace acc {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Vec {
+  static constexpr int N = 16 / (int)sizeof(T);
+  T e[N];
+};
+
+template <typename T> __device__ __forceinline__ Vec<T> load16(const T* p) {
+  u32x4 raw = *(const u32x4*)p;
+  Vec<T> v;
+  __builtin_memcpy(&v, &raw, 16);
+  return v;
+}
+
+// streaming load: the data is read once
+template <typename T> __device__ __forceinline__ Vec<T> load16_stream(const T* p) {
+  u32x4 raw = __builtin_nontemporal_load((const u32x4*)p);
+  Vec<T> v;
+  __builtin_memcpy(&v, &raw, 16);
+  return v;
+}
+
+template <typename T> __device__ __forceinline__ void store16(T* p, const Vec<T>& v) {
+  u32x4 raw;
+  __builtin_memcpy(&raw, &v, 16);
+  *(u32x4*)p = raw;
+}
+
+template <typename T> __device__ __forceinline__ void store16_stream(T* p, const Vec<T>& v) {
+  u32x4 raw;
+  __builtin_memcpy(&raw, &v, 16);
+  __builtin_nontemporal_store(raw, (u32x4*)p);
+}
+
+}  // namespace acc

@@ -1,0 +1,250 @@
+/*
+ * accelerate_hip.h - C ABI of the MI355X (gfx950) backend for Accelerate.
+ *
+ * This is the whole boundary between a host program (the Haskell backend
+ * package, a test, the serving loop) and the GPU.  It replaces every
+ * `Foreign.CUDA.*` call site of accelerate-llvm-ptx and adds the skeleton
+ * entry points that the `instance Execute` methods call.  Paths in the
+ * citations are relative to the accelerate-llvm repository:
+ *   PTX/  = accelerate-llvm-ptx/src/Data/Array/Accelerate/LLVM/PTX/
+ *   LLVM/ = accelerate-llvm/src/Data/Array/Accelerate/LLVM/
+ *   NAT/  = accelerate-llvm-native/src/Data/Array/Accelerate/LLVM/Native/
+ *
+ * Conventions
+ *   - Every function returns an acc_status (0 = success).  Nothing throws
+ *     across the ABI.  acc_hip_last_error() returns a thread-local message for
+ *     the last failure on the calling thread.
+ *   - Out of memory is ACC_ERROR_OUT_OF_MEMORY, distinct from other errors, so
+ *     a caller can evict and retry (PTX/Array/Remote.hs:63-74).
+ *   - Every call names its device (or a stream that belongs to one).  There is
+ *     no thread-bound "current context": calls are thread-safe and may come
+ *     from any OS thread (unlike PTX/Context.hs:135-139).
+ *   - Skeleton calls are asynchronous on the given stream.  Array arguments are
+ *     device pointers to struct-of-arrays fields (one buffer per scalar field,
+ *     LLVM/Execute/Marshal.hs:71-100); extents are 64-bit Accelerate `Int`s.
+ *   - Calls on one stream execute in order.  Two skeleton calls that run
+ *     concurrently must use different streams: each stream owns the small
+ *     device workspace (tile status words, tickets) its skeletons use.
+ */
+#ifndef ACCELERATE_HIP_H
+#define ACCELERATE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACC_HIP_ABI_VERSION 1
+
+typedef enum acc_status {
+  ACC_SUCCESS = 0,
+  ACC_ERROR_INVALID_VALUE = 1,
+  ACC_ERROR_OUT_OF_MEMORY = 2,
+  ACC_ERROR_HIP = 3,
+  ACC_ERROR_UNSUPPORTED = 4,
+  ACC_ERROR_EMPTY_ARRAY = 5, /* fold1 / scan-less reductions of an empty array (PTX/Execute.hs:232) */
+  ACC_ERROR_NOT_READY = 6,
+  ACC_ERROR_NO_DEVICE = 7
+} acc_status;
+
+/* Scalar element types (Accelerate SingleType tags). */
+typedef enum acc_type {
+  ACC_INT32 = 0,
+  ACC_INT64 = 1,
+  ACC_WORD32 = 2,
+  ACC_WORD64 = 3,
+  ACC_FLOAT32 = 4,
+  ACC_FLOAT64 = 5
+} acc_type;
+
+/* Combine functions available without run-time compilation.  They are the
+ * functions the reference recognises in permute (LLVM/CodeGen/Permute.hs:117-145)
+ * and the usual fold/scan operators.  Floating min/max are fminf/fmaxf
+ * (LLVM/CodeGen/Arithmetic.hs:456-466); integer arithmetic wraps. */
+typedef enum acc_op {
+  ACC_OP_ADD = 0,
+  ACC_OP_MUL = 1,
+  ACC_OP_MIN = 2,
+  ACC_OP_MAX = 3,
+  ACC_OP_AND = 4, /* integral types only */
+  ACC_OP_OR = 5,
+  ACC_OP_XOR = 6,
+  ACC_OP_SUB = 7,     /* permute only: old - new   (atomicrmw sub) */
+  ACC_OP_EXCHANGE = 8 /* permute only: new         (permute const)  */
+} acc_op;
+
+typedef enum acc_direction { ACC_LEFT_TO_RIGHT = 0, ACC_RIGHT_TO_LEFT = 1 } acc_direction;
+
+/* Stencil boundary conditions (LLVM/CodeGen/Stencil.hs:44-49, index maps :224-253). */
+typedef enum acc_boundary {
+  ACC_BOUNDARY_CLAMP = 0,
+  ACC_BOUNDARY_MIRROR = 1,
+  ACC_BOUNDARY_WRAP = 2,
+  ACC_BOUNDARY_CONSTANT = 3
+} acc_boundary;
+
+typedef struct acc_stream_s* acc_stream_t;
+typedef struct acc_event_s* acc_event_t;
+typedef struct acc_module_s* acc_module_t;
+typedef struct acc_graph_s* acc_graph_t;
+typedef void* acc_function_t;
+
+typedef struct acc_device_props {
+  char name[64];
+  char arch[32];            /* e.g. "gfx950" */
+  int compute_units;
+  int wave_size;
+  int max_threads_per_block;
+  size_t lds_per_block;
+  size_t total_global_mem;
+  int clock_khz;
+  int memory_clock_khz;
+  int memory_bus_width;
+  int l2_cache_bytes;
+} acc_device_props;
+
+/* ---------------------------------------------------------------- errors */
+const char* acc_hip_last_error(void);
+int acc_hip_abi_version(void);
+
+/* ----------------------------------------------------------------- device
+ * Replaces CUDA.count/device/props (PTX/State.hs:145-160) and deviceInfo
+ * (PTX/Context.hs:189-213).  Initialisation is implicit and per device. */
+int acc_hip_device_count(int* count);
+int acc_hip_device_props(int device, acc_device_props* props);
+int acc_hip_device_synchronize(int device);
+
+/* ----------------------------------------------------------------- memory
+ * Replaces CUDA.mallocArray/free/getMemInfo (PTX/Array/Remote.hs:63-100,
+ * PTX/Array/Table.hs:45-49).  The allocation limit is a test knob: with a
+ * limit set, an allocation that would take the library's live device bytes
+ * above it fails with ACC_ERROR_OUT_OF_MEMORY exactly as a full device does. */
+int acc_hip_malloc(int device, size_t bytes, void** ptr);
+int acc_hip_free(int device, void* ptr);
+int acc_hip_mem_get_info(int device, size_t* free_bytes, size_t* total_bytes);
+int acc_hip_set_alloc_limit(int device, size_t limit_bytes); /* 0 = no limit */
+int acc_hip_allocated_bytes(int device, size_t* bytes);
+/* pinned host memory: mallocHostForeignPtr (accelerate-llvm-ptx PTX.hs:548-553) */
+int acc_hip_host_alloc(size_t bytes, void** ptr);
+int acc_hip_host_free(void* ptr);
+
+/* Transfers: pokeArrayAsync, peekArrayAsync, copyArrayAsync, memsetAsync
+ * (PTX/Array/Prim.hs:125,159,188,217,295). */
+int acc_hip_memcpy_h2d_async(acc_stream_t s, void* dst, const void* src, size_t bytes);
+int acc_hip_memcpy_d2h_async(acc_stream_t s, void* dst, const void* src, size_t bytes);
+int acc_hip_memcpy_d2d_async(acc_stream_t s, void* dst, const void* src, size_t bytes);
+/* fill `count` elements of `elem_bytes` (1, 2, 4 or 8) with the bit pattern at `value` */
+int acc_hip_fill_async(acc_stream_t s, void* dst, const void* value, size_t elem_bytes, size_t count);
+
+/* ---------------------------------------------------------------- streams
+ * Stream.create / Stream.finished (PTX/Execute/Stream.hs:125,
+ * PTX/Execute/Stream/Reservoir.hs:73).  Streams are created at the default
+ * priority. */
+int acc_hip_stream_create(int device, acc_stream_t* s);
+int acc_hip_stream_destroy(acc_stream_t s);
+int acc_hip_stream_synchronize(acc_stream_t s);
+int acc_hip_stream_query(acc_stream_t s); /* ACC_SUCCESS = idle, ACC_ERROR_NOT_READY = busy */
+int acc_hip_stream_device(acc_stream_t s, int* device);
+/* Pre-size the stream's skeleton workspace (bytes of tile-status words), so that
+ * no allocation happens later, e.g. while the stream is being captured. */
+int acc_hip_stream_reserve(acc_stream_t s, size_t status_bytes);
+
+/* ----------------------------------------------------------------- events
+ * Event.create/record/wait/query/block/elapsedTime
+ * (PTX/Execute/Event.hs:65,117,129,138,144; PTX/Debug.hs:62-83). */
+int acc_hip_event_create(int device, int timing, acc_event_t* e);
+int acc_hip_event_destroy(acc_event_t e);
+int acc_hip_event_record(acc_event_t e, acc_stream_t s);
+int acc_hip_stream_wait_event(acc_stream_t s, acc_event_t e);
+int acc_hip_event_query(acc_event_t e); /* ACC_SUCCESS = complete, ACC_ERROR_NOT_READY */
+int acc_hip_event_synchronize(acc_event_t e);
+int acc_hip_event_elapsed_ms(acc_event_t start, acc_event_t stop, float* ms);
+
+/* ---------------------------------------------------------- code objects
+ * loadDataFromPtrEx, unload, getFun, requires (PTX/Link.hs:64,75,101-106;
+ * PTX/Embed.hs:55,77) and CUDA.launchKernel (PTX/Execute.hs:864). */
+int acc_hip_module_load_data(int device, const void* image, size_t bytes, acc_module_t* m);
+int acc_hip_module_unload(acc_module_t m);
+int acc_hip_module_get_function(acc_module_t m, const char* name, acc_function_t* f);
+/* registers, static LDS bytes and max threads per block of a function */
+int acc_hip_function_attributes(acc_function_t f, int* num_regs, int* static_lds_bytes,
+                                int* max_threads);
+int acc_hip_launch(acc_function_t f, unsigned gx, unsigned gy, unsigned gz, unsigned bx,
+                   unsigned by, unsigned bz, unsigned dynamic_lds_bytes, acc_stream_t s,
+                   void** args);
+
+/* ----------------------------------------------------------------- graphs
+ * Stream capture into a hipGraph (new: the batched form). */
+int acc_hip_graph_begin_capture(acc_stream_t s);
+int acc_hip_graph_end_capture(acc_stream_t s, acc_graph_t* g);
+int acc_hip_graph_launch(acc_graph_t g, acc_stream_t s);
+int acc_hip_graph_destroy(acc_graph_t g);
+
+/* ============================================================== skeletons
+ * One entry point per `class Execute` method on the hot path
+ * (LLVM/Execute.hs:58-163), reproducing the dispatch of `instance Execute PTX`
+ * (PTX/Execute.hs:83-109): output sizes, empty-array cases and clone versus
+ * in-place.  `seed` points to one host element of the array's type, or is NULL
+ * for the seedless variant (fold1, scanl1, ...).  All array pointers are
+ * device pointers.  An array of shape (sh :. n) is passed as outer = size sh,
+ * inner = n, row-major. */
+
+/* fold / fold1 (LLVM/Execute.hs:87-92; PTX/Execute.hs:222-340).
+ * out[r] = seed (+) (in[r][0] (+) ... (+) in[r][inner-1]) for r < outer; the
+ * seed is combined on the left.  inner == 0 with a seed fills `out` with the
+ * seed; without a seed it is ACC_ERROR_EMPTY_ARRAY. */
+int acc_hip_fold(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void* in,
+                 int64_t outer, int64_t inner, void* out);
+
+/* foldSeg / fold1Seg (LLVM/Execute.hs:94-102; PTX/Execute.hs:343-374).
+ * `offsets` (nseg + 1 values of type ACC_INT32 or ACC_INT64) are the
+ * `scanl (+) 0` of the segment lengths, relative to the start of each row.
+ * out[r][s] = seed (+) fold over in[r][offsets[s] .. offsets[s+1]).  An empty
+ * segment yields the seed; fold1Seg requires non-empty segments. */
+int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void* in,
+                     int64_t outer, int64_t inner, acc_type offset_type, const void* offsets,
+                     int64_t nseg, void* out);
+
+/* scanl1/scanr1 (seed == NULL, out has inner elements per row) and
+ * scanl/scanr (seed != NULL, out has inner + 1 elements per row)
+ * (LLVM/Execute.hs:104-111; PTX/Execute.hs:377-488; element order
+ * NAT/CodeGen/Scan.hs:136-210). */
+int acc_hip_scan(acc_stream_t s, acc_type t, acc_op op, acc_direction dir, const void* seed,
+                 const void* in, int64_t outer, int64_t inner, void* out);
+
+/* scanl'/scanr' (LLVM/Execute.hs:113-119; PTX/Execute.hs:491-594;
+ * NAT/CodeGen/Scan.hs:213-270): out has inner elements per row (the exclusive
+ * prefixes), sums has outer elements (the totals). */
+int acc_hip_scan_prime(acc_stream_t s, acc_type t, acc_op op, acc_direction dir,
+                       const void* seed, const void* in, int64_t outer, int64_t inner,
+                       void* out, void* sums);
+
+/* permute (LLVM/Execute.hs:121-129; PTX/Execute.hs:597-649;
+ * PTX/CodeGen/Permute.hs:117-258).
+ *   out[index[i]] = combine(src[i], out[index[i]])  for i < n with
+ *   0 <= index[i] < m; other indices are `Nothing` (the element is dropped).
+ * If out != defaults, defaults is first copied to out (clone); if they are
+ * equal the update is in place.  src == NULL means a delayed constant source
+ * (`fill sh v`) whose value is *src_fill (the histogram case costs no source
+ * bytes).  index_type is ACC_INT32 or ACC_INT64. */
+int acc_hip_permute(acc_stream_t s, acc_type t, acc_op combine, const void* defaults, void* out,
+                    int64_t m, const void* src, const void* src_fill, acc_type index_type,
+                    const void* index, int64_t n);
+
+/* stencil1 on a DIM2 array (LLVM/Execute.hs:131-138; PTX/Execute.hs:652-776)
+ * with the stencil function f(tuple) = sum over taps of w[dy][dx] * x[y+dy][x+dx],
+ * taps in row-major order (LLVM/CodeGen/Stencil.hs:126-180), dy in [-ry, ry],
+ * dx in [-rx, rx]; `weights` is a host array of (2ry+1)*(2rx+1) elements of
+ * type t.  Radii 1..4 (3/5/7/9-point patterns).  `constant` is used with
+ * ACC_BOUNDARY_CONSTANT. */
+int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, const void* weights,
+                      acc_boundary boundary, const void* constant, const void* in,
+                      int64_t height, int64_t width, void* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACCELERATE_HIP_H */

@@ -1,0 +1,248 @@
+"""numpy front end of liboracle.so (the C restatement in oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import c_double, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_SUFFIX = {
+    np.dtype(np.int32): "i32",
+    np.dtype(np.int64): "i64",
+    np.dtype(np.uint32): "u32",
+    np.dtype(np.uint64): "u64",
+    np.dtype(np.float32): "f32",
+    np.dtype(np.float64): "f64",
+}
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        for name in ("or_native_fold_add_i32",):
+            getattr(_lib, name).restype = c_int32
+        _lib.or_divide_work1.restype = c_int64
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def _seed(dtype, seed):
+    if seed is None:
+        return None, None
+    a = np.array([seed], dtype=dtype)
+    return a, _ptr(a)
+
+
+def _check(rc, what):
+    if rc == -2:
+        raise ValueError(f"{what}: empty array")
+    if rc != 0:
+        raise ValueError(f"{what}: unsupported combine function")
+
+
+def fold(op, seed, x, outer=1):
+    x = np.ascontiguousarray(x)
+    inner = x.size // outer if outer else 0
+    out = np.empty(outer, x.dtype)
+    keep, sp = _seed(x.dtype, seed)
+    f = getattr(lib(), "or_fold_" + _SUFFIX[x.dtype])
+    _check(f(c_int(op), sp, _ptr(x), c_int64(outer), c_int64(inner), _ptr(out)), "fold")
+    return out
+
+
+def fold_seg(op, seed, x, offsets, outer=1):
+    x = np.ascontiguousarray(x)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nseg = off.size - 1
+    inner = x.size // outer if outer else 0
+    out = np.zeros(outer * nseg, x.dtype)
+    keep, sp = _seed(x.dtype, seed)
+    f = getattr(lib(), "or_fold_seg_" + _SUFFIX[x.dtype])
+    _check(f(c_int(op), sp, _ptr(x), c_int64(outer), c_int64(inner), _ptr(off), c_int64(nseg), _ptr(out)), "foldSeg")
+    return out
+
+
+def scan(op, direction, seed, x, outer=1):
+    x = np.ascontiguousarray(x)
+    inner = x.size // outer if outer else 0
+    on = inner + 1 if seed is not None else inner
+    out = np.empty(outer * on, x.dtype)
+    keep, sp = _seed(x.dtype, seed)
+    f = getattr(lib(), "or_scan_" + _SUFFIX[x.dtype])
+    _check(f(c_int(op), c_int(direction), sp, _ptr(x), c_int64(outer), c_int64(inner), _ptr(out)), "scan")
+    return out
+
+
+def scan_prime(op, direction, seed, x, outer=1):
+    x = np.ascontiguousarray(x)
+    inner = x.size // outer if outer else 0
+    out = np.empty(outer * inner, x.dtype)
+    sums = np.empty(outer, x.dtype)
+    keep, sp = _seed(x.dtype, seed)
+    f = getattr(lib(), "or_scan_prime_" + _SUFFIX[x.dtype])
+    _check(f(c_int(op), c_int(direction), sp, _ptr(x), c_int64(outer), c_int64(inner), _ptr(out), _ptr(sums)),
+           "scan'")
+    return out, sums
+
+
+def permute(op, defaults, src, fill, idx):
+    out = np.array(defaults, copy=True)
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    s = None if src is None else np.ascontiguousarray(src, dtype=out.dtype)
+    keep, fp = _seed(out.dtype, fill)
+    f = getattr(lib(), "or_permute_" + _SUFFIX[out.dtype])
+    _check(f(c_int(op), _ptr(out), c_int64(out.size), _ptr(s), fp, _ptr(idx), c_int64(idx.size)), "permute")
+    return out
+
+
+def stencil2d(ry, rx, weights, boundary, constant, img):
+    img = np.ascontiguousarray(img)
+    h, w = img.shape
+    wt = np.ascontiguousarray(np.asarray(weights, dtype=img.dtype).reshape(-1))
+    out = np.empty_like(img)
+    keep, cp = _seed(img.dtype, constant if constant is not None else 0)
+    f = getattr(lib(), "or_stencil2d_" + _SUFFIX[img.dtype])
+    _check(f(c_int(ry), c_int(rx), _ptr(wt), c_int(boundary), cp, _ptr(img), c_int64(h), c_int64(w), _ptr(out)),
+           "stencil")
+    return out
+
+
+# ---- FP64 truth for Float32 data
+def fold_f64(x, seed=None):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    o, a = c_double(), c_double()
+    keep, sp = _seed(np.float32, seed)
+    lib().or_fold_f32_f64(_ptr(x), c_int64(x.size), sp, ctypes.byref(o), ctypes.byref(a))
+    return o.value, a.value
+
+
+def fold_seg_f64(x, offsets, seed=None):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nseg = off.size - 1
+    out = np.empty(nseg, np.float64)
+    ab = np.empty(nseg, np.float64)
+    keep, sp = _seed(np.float32, seed)
+    lib().or_fold_seg_f32_f64(_ptr(x), _ptr(off), c_int64(nseg), sp, _ptr(out), _ptr(ab))
+    return out, ab
+
+
+def scan_f64(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.size, np.float64)
+    ab = np.empty(x.size, np.float64)
+    lib().or_scan_f32_f64(_ptr(x), c_int64(x.size), _ptr(out), _ptr(ab))
+    return out, ab
+
+
+def stencil2d_f64(ry, rx, weights, boundary, constant, img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    h, w = img.shape
+    wt = np.ascontiguousarray(np.asarray(weights, dtype=np.float32).reshape(-1))
+    out = np.empty((h, w), np.float64)
+    ab = np.empty((h, w), np.float64)
+    keep, cp = _seed(np.float32, constant if constant is not None else 0)
+    lib().or_stencil2d_f32_f64(c_int(ry), c_int(rx), _ptr(wt), c_int(boundary), cp, _ptr(img), c_int64(h),
+                               c_int64(w), _ptr(out), _ptr(ab))
+    return out, ab
+
+
+def divide_work1(splits, minsize, lo, hi):
+    cap = 4 * max(splits, 1) + 16
+    r = np.empty(2 * cap, np.int64)
+    n = lib().or_divide_work1(c_int64(splits), c_int64(minsize), c_int64(lo), c_int64(hi), _ptr(r), c_int64(cap))
+    return [(int(r[2 * i]), int(r[2 * i + 1])) for i in range(min(n, cap))]
+
+
+# ---- native-schedule parallel versions (CPU baseline)
+def native_fold_add_i32(threads, x, seed=0):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    return int(lib().or_native_fold_add_i32(c_int(threads), _ptr(x), c_int64(x.size), c_int32(seed)))
+
+
+def native_scanl1_add_i32(threads, x, out=None):
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    if out is None:
+        out = np.empty_like(x)
+    lib().or_native_scanl1_add_i32(c_int(threads), _ptr(x), c_int64(x.size), _ptr(out))
+    return out
+
+
+def native_fold1_seg_f32(threads, op, x, offsets, out=None):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    if out is None:
+        out = np.empty(off.size - 1, np.float32)
+    lib().or_native_fold1_seg_f32(c_int(threads), c_int(op), _ptr(x), _ptr(off), c_int64(off.size - 1), _ptr(out))
+    return out
+
+
+def native_histogram_i32(threads, idx, m, bins=None):
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    if bins is None:
+        bins = np.zeros(m, np.int32)
+    lib().or_native_histogram_i32(c_int(threads), _ptr(idx), c_int64(idx.size), _ptr(bins), c_int64(m))
+    return bins
+
+
+def native_stencil2d_f32(threads, ry, rx, weights, boundary, img, out=None):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    h, w = img.shape
+    wt = np.ascontiguousarray(np.asarray(weights, dtype=np.float32).reshape(-1))
+    if out is None:
+        out = np.empty_like(img)
+    lib().or_native_stencil2d_f32(c_int(threads), c_int(ry), c_int(rx), _ptr(wt), c_int(boundary), _ptr(img),
+                                  c_int64(h), c_int64(w), _ptr(out))
+    return out
+
+
+# ---- seeded generators
+def gen_uniform_i32(seed, lo, hi, n):
+    out = np.empty(n, np.int32)
+    lib().or_gen_uniform_i32(c_uint64(seed), c_int32(lo), c_int32(hi), _ptr(out), c_int64(n))
+    return out
+
+
+def gen_uniform_f32(seed, lo, hi, n):
+    out = np.empty(n, np.float32)
+    lib().or_gen_uniform_f32(c_uint64(seed), c_float(lo), c_float(hi), _ptr(out), c_int64(n))
+    return out
+
+
+def gen_normal_f32(seed, n):
+    out = np.empty(n, np.float32)
+    lib().or_gen_normal_f32(c_uint64(seed), _ptr(out), c_int64(n))
+    return out
+
+
+def gen_zipf_i32(seed, s, nbins, n):
+    out = np.empty(n, np.int32)
+    lib().or_gen_zipf_i32(c_uint64(seed), c_double(s), c_int32(nbins), _ptr(out), c_int64(n))
+    return out
+
+
+def gen_segment_lengths(seed, nseg, maxlen, total):
+    out = np.empty(nseg, np.int64)
+    lib().or_gen_segment_lengths(c_uint64(seed), c_int64(nseg), c_int64(maxlen), c_int64(total), _ptr(out))
+    return out

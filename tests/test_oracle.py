@@ -1,0 +1,297 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each check comes from outside the oracle's own formulas: the known-answer
+values written in the reference sources, numpy / scipy library routines,
+closed forms and brute force.  A plausible slip in the restatement (wrong
+direction, seed on the wrong side, off-by-one offsets, mirrored boundary)
+fails at least one of them.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+ADD, MUL, MIN, MAX, AND, OR, XOR, SUB, EXCHANGE = range(9)
+L2R, R2L = 0, 1
+CLAMP, MIRROR, WRAP, CONSTANT = range(4)
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_examples.json")))
+
+
+# ------------------------------------------------ reference known answers
+def test_runq_known_answer():
+    g = GOLDEN["runq_sum_map_plus1"]
+    x = np.arange(1, 10001, dtype=np.int64) + 1  # map (+1) [1..10000]
+    assert O.fold(ADD, 0, x)[0] == g["expect"]
+
+
+def test_scanl_doc_example():
+    g = GOLDEN["scanl_plus_10"]
+    out = O.scan(ADD, L2R, g["seed"], np.array(g["input"], np.int64))
+    assert out.tolist() == g["expect"]
+
+
+def test_scanr_prime_code_example():
+    g = GOLDEN["scanr_prime_plus_10"]
+    out, s = O.scan_prime(ADD, R2L, g["seed"], np.array(g["input"], np.int64))
+    assert out.tolist() == g["expect"] and s[0] == g["expect_sum"]
+
+
+def test_scanl_prime_doc_example():
+    g = GOLDEN["scanl_prime_plus_10"]
+    out, s = O.scan_prime(ADD, L2R, g["seed"], np.array(g["input"], np.int64))
+    assert out.tolist() == g["expect"] and s[0] == g["expect_sum"]
+
+
+# ------------------------------------------------------- fold / scan vs numpy
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint32, np.uint64])
+def test_fold_add_wraps_like_numpy(dt):
+    rng = np.random.default_rng(1)
+    info = np.iinfo(dt)
+    x = rng.integers(info.min // 2, info.max // 2, size=5000, dtype=dt, endpoint=True)
+    assert O.fold(ADD, None, x)[0] == np.sum(x, dtype=dt)
+    assert O.fold(ADD, dt(7), x)[0] == dt(np.sum(x, dtype=dt) + dt(7))
+    assert O.fold(MAX, None, x)[0] == x.max()
+    assert O.fold(MIN, None, x)[0] == x.min()
+    assert O.fold(XOR, None, x)[0] == np.bitwise_xor.reduce(x)
+    assert O.fold(AND, None, x)[0] == np.bitwise_and.reduce(x)
+    assert O.fold(OR, None, x)[0] == np.bitwise_or.reduce(x)
+
+
+def test_fold_rows_and_empty():
+    x = np.arange(12, dtype=np.int32).reshape(3, 4)
+    assert O.fold(ADD, 100, x, outer=3).tolist() == [106, 122, 138]
+    assert O.fold(ADD, 5, np.zeros(0, np.int32), outer=1).tolist() == [5]
+    with pytest.raises(ValueError):
+        O.fold(ADD, None, np.zeros(0, np.int32), outer=1)
+
+
+def test_fold1_closed_forms():
+    n = 1 << 16
+    assert O.fold(ADD, None, np.arange(1, n + 1, dtype=np.int64))[0] == n * (n + 1) // 2
+    alt = np.where(np.arange(n) % 2 == 0, 1, -1).astype(np.int32)
+    assert O.fold(ADD, None, alt)[0] == 0
+    assert O.fold(ADD, None, np.full(n, 3, np.int32))[0] == 3 * n
+    # wrap-around: 2^31 ones in int32 arithmetic would overflow; use (2^31-1) + 1
+    assert O.fold(ADD, None, np.array([2**31 - 1, 1], np.int32))[0] == -(2**31)
+
+
+def test_fold_float_minmax_nan_semantics():
+    # fmaxf/fminf drop a NaN operand (C99 7.12.12)
+    x = np.array([np.nan, 1.0, np.nan, -3.0], np.float32)
+    assert O.fold(MAX, None, x)[0] == 1.0
+    assert O.fold(MIN, None, x)[0] == -3.0
+    assert np.isnan(O.fold(MAX, None, np.array([np.nan], np.float32))[0])
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint64])
+def test_scan_matches_cumsum(dt):
+    rng = np.random.default_rng(2)
+    info = np.iinfo(dt)
+    x = rng.integers(info.min // 4, info.max // 4, size=3001, dtype=dt, endpoint=True)
+    c = np.cumsum(x, dtype=dt)
+    assert np.array_equal(O.scan(ADD, L2R, None, x), c)                       # scanl1
+    assert np.array_equal(O.scan(ADD, L2R, dt(0), x), np.concatenate([np.zeros(1, dt), c]))  # scanl
+    rc = np.cumsum(x[::-1], dtype=dt)[::-1]
+    assert np.array_equal(O.scan(ADD, R2L, None, x), rc)                      # scanr1
+    assert np.array_equal(O.scan(ADD, R2L, dt(0), x), np.concatenate([rc, np.zeros(1, dt)]))  # scanr
+    e, s = O.scan_prime(ADD, L2R, dt(0), x)
+    assert np.array_equal(e, np.concatenate([np.zeros(1, dt), c[:-1]])) and s[0] == c[-1]
+    e, s = O.scan_prime(ADD, R2L, dt(0), x)
+    assert np.array_equal(e, np.concatenate([rc[1:], np.zeros(1, dt)])) and s[0] == rc[0]
+
+
+def test_scan_invariants_max():
+    rng = np.random.default_rng(3)
+    x = rng.integers(-1000, 1000, size=777, dtype=np.int32)
+    s1 = O.scan(MAX, L2R, None, x)
+    assert np.array_equal(s1, np.maximum.accumulate(x))
+    assert s1[-1] == O.fold(MAX, None, x)[0]  # last (scanl1 f xs) == fold1 f xs
+    # scanl z xs == scanl1 on (z : xs)
+    z = np.int32(-5000)
+    assert np.array_equal(O.scan(MAX, L2R, z, x), O.scan(MAX, L2R, None, np.concatenate([[z], x]).astype(np.int32)))
+
+
+def test_scan_rows():
+    x = np.arange(12, dtype=np.int32).reshape(3, 4)
+    out = O.scan(ADD, L2R, None, x, outer=3).reshape(3, 4)
+    assert np.array_equal(out, np.cumsum(x, axis=1, dtype=np.int32))
+    out = O.scan(ADD, L2R, np.int32(1), x, outer=3).reshape(3, 5)
+    assert out[:, 0].tolist() == [1, 1, 1] and np.array_equal(out[:, 1:], 1 + np.cumsum(x, axis=1))
+
+
+def test_scan_empty():
+    assert O.scan(ADD, L2R, np.int32(4), np.zeros(0, np.int32)).tolist() == [4]
+    assert O.scan(ADD, L2R, None, np.zeros(0, np.int32)).tolist() == []
+    e, s = O.scan_prime(ADD, R2L, np.int32(9), np.zeros(0, np.int32))
+    assert e.tolist() == [] and s.tolist() == [9]
+
+
+# ------------------------------------------------------------------ foldSeg
+def test_fold_seg_matches_reduceat():
+    rng = np.random.default_rng(4)
+    lens = rng.integers(1, 50, size=300)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = rng.integers(-100, 100, size=int(off[-1]), dtype=np.int32)
+    assert np.array_equal(O.fold_seg(ADD, None, x, off), np.add.reduceat(x, off[:-1]).astype(np.int32))
+    assert np.array_equal(O.fold_seg(MAX, None, x, off), np.maximum.reduceat(x, off[:-1]))
+    assert np.array_equal(O.fold_seg(ADD, np.int32(10), x, off), 10 + np.add.reduceat(x, off[:-1]))
+
+
+def test_fold_seg_empty_segments_give_seed():
+    x = np.arange(10, dtype=np.int64)
+    off = np.array([0, 3, 3, 7, 10, 10], np.int64)
+    assert O.fold_seg(ADD, np.int64(-1), x, off).tolist() == [2, -1, 17, 23, -1]
+
+
+def test_fold_seg_rows():
+    x = np.arange(20, dtype=np.int32).reshape(2, 10)
+    off = np.array([0, 4, 10], np.int64)
+    assert O.fold_seg(ADD, None, x, off, outer=2).tolist() == [6, 39, 46, 99]
+
+
+def test_fp64_truth():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(20000).astype(np.float32)
+    v, a = O.fold_f64(x)
+    assert v == pytest.approx(math.fsum(x.astype(np.float64)), abs=1e-12)
+    assert a == pytest.approx(math.fsum(np.abs(x.astype(np.float64))), rel=1e-15)
+    off = np.array([0, 5, 5, 19999, 20000], np.int64)
+    sv, sa = O.fold_seg_f64(x, off, seed=np.float32(2.0))
+    for s in range(4):
+        assert sv[s] == pytest.approx(2.0 + math.fsum(x[off[s]:off[s + 1]].astype(np.float64)), abs=1e-12)
+    c, _ = O.scan_f64(x[:1000])
+    assert c[999] == pytest.approx(math.fsum(x[:1000].astype(np.float64)), abs=1e-12)
+
+
+# ------------------------------------------------------------------ permute
+def test_histogram_is_bincount():
+    idx = O.gen_zipf_i32(42, 1.1, 4096, 200000)
+    bins = O.permute(ADD, np.zeros(4096, np.int32), None, 1, idx)
+    assert np.array_equal(bins, np.bincount(idx, minlength=4096).astype(np.int32))
+    assert bins.sum() == idx.size
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.float64])
+def test_permute_ops_brute_force(dt):
+    rng = np.random.default_rng(6)
+    m, n = 97, 5000
+    idx = rng.integers(-5, m + 5, size=n)  # out-of-range indices are Nothing
+    src = rng.integers(-50, 50, size=n).astype(dt)
+    d = rng.integers(-50, 50, size=m).astype(dt)
+    ok = (idx >= 0) & (idx < m)
+    for op, ufunc in ((ADD, np.add), (MIN, np.minimum), (MAX, np.maximum)):
+        want = d.copy()
+        ufunc.at(want, idx[ok], src[ok])
+        assert np.array_equal(O.permute(op, d, src, None, idx), want)
+    want = d.copy()
+    np.subtract.at(want, idx[ok], src[ok])
+    assert np.array_equal(O.permute(SUB, d, src, None, idx), want)
+
+
+def test_permute_exchange_injective():
+    idx = np.array([3, 0, 2, -1], np.int64)
+    out = O.permute(EXCHANGE, np.full(4, 9, np.int32), np.array([10, 11, 12, 13], np.int32), None, idx)
+    assert out.tolist() == [11, 9, 12, 10]
+
+
+# ------------------------------------------------------------------ stencil
+def _scipy_correlate(img, w, mode, cval=0.0):
+    ndimage = pytest.importorskip("scipy.ndimage")
+    # Accelerate boundaries as scipy modes: clamp = nearest (aaa|abc|ccc),
+    # mirror = mirror (cb|abc|ba, edge not repeated), wrap = wrap
+    m = {CLAMP: "nearest", MIRROR: "mirror", WRAP: "wrap", CONSTANT: "constant"}[mode]
+    return ndimage.correlate(img.astype(np.float64), w.astype(np.float64), mode=m, cval=cval)
+
+
+@pytest.mark.parametrize("mode", [CLAMP, MIRROR, WRAP, CONSTANT])
+@pytest.mark.parametrize("r", [(1, 1), (2, 2), (1, 3)])
+def test_stencil_matches_scipy(mode, r):
+    ry, rx = r
+    rng = np.random.default_rng(7)
+    img = rng.standard_normal((37, 53))
+    w = rng.standard_normal((2 * ry + 1, 2 * rx + 1))
+    out = O.stencil2d(ry, rx, w, mode, 0.5, img)
+    np.testing.assert_allclose(out, _scipy_correlate(img, w, mode, 0.5), rtol=1e-12, atol=1e-12)
+    v, _ = O.stencil2d_f64(ry, rx, w.astype(np.float32), mode, 0.5, img.astype(np.float32))
+    np.testing.assert_allclose(v, _scipy_correlate(img.astype(np.float32), w.astype(np.float32), mode, 0.5),
+                               rtol=1e-9, atol=1e-9)
+
+
+def test_stencil_special_cases():
+    w5 = np.outer([1, 4, 6, 4, 1], [1, 4, 6, 4, 1]).astype(np.float64) / 256
+    c = O.stencil2d(2, 2, w5, CLAMP, None, np.full((16, 16), 3.0))
+    assert np.allclose(c, 3.0 * w5.sum())
+    imp = np.zeros((9, 9))
+    imp[4, 4] = 1.0
+    w = np.arange(25, dtype=np.float64).reshape(5, 5)
+    out = O.stencil2d(2, 2, w, CONSTANT, 0.0, imp)
+    # correlation of an impulse gives the weights mirrored about the centre
+    assert np.array_equal(out[2:7, 2:7], w[::-1, ::-1])
+
+
+def test_boundary_maps_by_hand():
+    # 1-D image [0,1,2,3,4] with a 3-point stencil picking only the left tap
+    img = np.arange(5, dtype=np.float64).reshape(1, 5)
+    wl = np.zeros((3, 3))
+    wl[1, 0] = 1.0  # x - 1
+    wr = np.zeros((3, 3))
+    wr[1, 2] = 1.0  # x + 1
+    assert O.stencil2d(1, 1, wl, CLAMP, None, img)[0, 0] == 0.0
+    assert O.stencil2d(1, 1, wl, MIRROR, None, img)[0, 0] == 1.0   # -1 -> 1
+    assert O.stencil2d(1, 1, wl, WRAP, None, img)[0, 0] == 4.0     # -1 -> n-1
+    assert O.stencil2d(1, 1, wl, CONSTANT, 7.0, img)[0, 0] == 7.0
+    assert O.stencil2d(1, 1, wr, CLAMP, None, img)[0, 4] == 4.0
+    assert O.stencil2d(1, 1, wr, MIRROR, None, img)[0, 4] == 3.0   # n -> n-2
+    assert O.stencil2d(1, 1, wr, WRAP, None, img)[0, 4] == 0.0
+
+
+def test_integer_stencil_exact():
+    rng = np.random.default_rng(8)
+    img = rng.integers(-1000, 1000, size=(20, 30)).astype(np.int32)
+    w = np.outer([1, 2, 1], [1, 2, 1]).astype(np.int32)
+    out = O.stencil2d(1, 1, w, MIRROR, None, img)
+    want = _scipy_correlate(img, w, MIRROR).astype(np.int64)
+    assert np.array_equal(out.astype(np.int64), want)
+
+
+# --------------------------------------------------------------- schedule
+def test_divide_work1_pieces():
+    for splits, minsize, n in ((7, 4096, 1 << 20), (3, 8192, 100000), (15, 4096, 5000), (0, 4096, 123)):
+        p = O.divide_work1(splits, minsize, 0, n)
+        assert p[0][0] == 0 and p[-1][1] == n
+        assert all(a[1] == b[0] for a, b in zip(p, p[1:]))
+        assert len(p) <= splits + 1
+        assert all(u % minsize == 0 for u, _ in p)
+    assert O.divide_work1(1, 4096, 0, 10000) == [(0, 8192), (8192, 10000)]
+
+
+def test_native_schedule_matches_sequential():
+    x = O.gen_uniform_i32(42, -1000, 1000, 300000)
+    assert O.native_fold_add_i32(4, x) == O.fold(ADD, 0, x)[0]
+    assert np.array_equal(O.native_scanl1_add_i32(4, x), O.scan(ADD, L2R, None, x))
+    idx = O.gen_zipf_i32(1, 1.1, 4096, 100000)
+    assert np.array_equal(O.native_histogram_i32(4, idx, 4096), np.bincount(idx, minlength=4096))
+    lens = O.gen_segment_lengths(3, 500, 1000, 100000)
+    off = np.concatenate([[0], np.cumsum(lens)])
+    v = O.gen_normal_f32(9, 100000)
+    assert np.array_equal(O.native_fold1_seg_f32(4, MAX, v, off), O.fold_seg(MAX, None, v, off))
+    img = O.gen_normal_f32(5, 64 * 80).reshape(64, 80)
+    w = np.outer([1, 2, 1], [1, 2, 1]).astype(np.float32) / 16
+    np.testing.assert_allclose(O.native_stencil2d_f32(4, 1, 1, w, MIRROR, img), O.stencil2d(1, 1, w, MIRROR, None, img),
+                               rtol=1e-5, atol=1e-6)
+
+
+def test_generators():
+    lens = O.gen_segment_lengths(42, 1 << 12, 16384, 1 << 22)
+    assert lens.sum() == 1 << 22 and lens.min() >= 1
+    z = O.gen_zipf_i32(42, 1.1, 4096, 1 << 20)
+    p1 = 1.0 / sum(k ** -1.1 for k in range(1, 4097))
+    assert abs((z == 0).mean() - p1) < 0.01
+    u = O.gen_uniform_i32(42, -1000, 1000, 1 << 16)
+    assert u.min() == -1000 and u.max() == 1000
+    assert np.array_equal(u, O.gen_uniform_i32(42, -1000, 1000, 1 << 16))

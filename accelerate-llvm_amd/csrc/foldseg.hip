@@ -44,25 +44,34 @@ template <typename Op> struct SegOp {
 };
 
 // Segment starts given by an offsets array: start(g) = off[g], g in [0, S].
+// The segment range of each tile comes from a pre-pass (k_tile_seg).
 template <typename I> struct OffsetStarts {
   const I* off;
+  const int64_t* tseg;  // tseg[T] = last g with start(g) <= T * TILE (>= 0)
   __device__ __forceinline__ int64_t start(int64_t g) const { return (int64_t)off[g]; }
+  __device__ __forceinline__ void tile_range(uint32_t t, int64_t, int64_t, int64_t, int64_t* lo, int64_t* hi) const {
+    *lo = tseg[t];
+    *hi = tseg[t + 1];  // an upper bound of the segment of the tile's last element
+  }
 };
 
 // foldDim: start(g) = g * len.
 struct UniformStarts {
   int64_t len;
   __device__ __forceinline__ int64_t start(int64_t g) const { return g * len; }
+  __device__ __forceinline__ void tile_range(uint32_t, int64_t first, int64_t last, int64_t nseg, int64_t* lo,
+                                             int64_t* hi) const {
+    *lo = first / len;
+    *hi = min(last / len, nseg);
+  }
 };
 
 template <typename T, typename Starts> struct SegArgs {
   const T* in;
   T* out;
   Starts st;
-  int64_t nseg;    // S
-  int64_t e_begin; // start(0)
-  int64_t e_end;   // start(S)
-  int64_t tile0;   // first tile index (e_begin / TILE)
+  int64_t nseg;   // S
+  int64_t e_end;  // elements per row
   uint32_t ntiles;
   uint32_t has_seed;
   T seed;
@@ -71,19 +80,36 @@ template <typename T, typename Starts> struct SegArgs {
   uint64_t* status;
 };
 
-template <typename Starts>
-__device__ __forceinline__ int64_t seg_search(const Starts& st, int64_t lo, int64_t hi, int64_t e) {
-  // last g in [lo, hi] with start(g) <= e; lo - 1 if none
-  int64_t a = lo, b = hi + 1;  // invariant: start(a-1) <= e (virtual), start(b) > e
-  while (a < b) {
-    int64_t m = (a + b) >> 1;
-    if (st.start(m) <= e) a = m + 1; else b = m;
+// One wave per tile boundary T in [0, ntiles]: tseg[T] = last g in [0, S]
+// with off[g] <= T * tile (0 if none), by a 64-ary search (one round trip
+// narrows the range 64-fold).
+template <typename I>
+__global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, int64_t tile, uint32_t ntiles,
+                                                  int64_t* tseg) {
+  const uint32_t T = blockIdx.x * 4u + threadIdx.x / 64u;
+  if (T > ntiles) return;  // whole waves
+  const int lane = lane_id();
+  const int64_t e = (int64_t)T * tile;
+  int64_t lo = 0, hi = nseg;
+  if ((int64_t)off[0] > e) {
+    hi = 0;  // no segment starts at or before e
+  } else {
+    while (hi > lo) {
+      const int64_t step = (hi - lo + 64) / 64;
+      const int64_t g = lo + (int64_t)lane * step;
+      const bool le = g <= hi && (int64_t)off[g] <= e;
+      const int c = __popcll(__ballot(le));  // >= 1: off[lo] <= e
+      lo = lo + (int64_t)(c - 1) * step;
+      hi = min(hi, lo + step - 1);
+    }
   }
-  return a - 1;
+  if (lane == 0) tseg[T] = lo;
 }
 
 template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
-__global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
+__device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t, uint32_t epoch,
+                                         int64_t* s_range, int64_t* s_starts, Maybe<SegV<T>>* s_wave,
+                                         Maybe<SegV<T>>* s_prefix) {
   constexpr int VEC = Vec<T>::N;
   constexpr int W = BLOCK / 64;
   constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
@@ -91,29 +117,17 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   constexpr int NW = Tagged<S>::NW;
   SegOp<Op> sop{Op{}};
   Op op{};
-  __shared__ uint32_t s_u32[2];
-  __shared__ int64_t s_range[2];
-  __shared__ int64_t s_starts[CAP];
-  __shared__ Maybe<S> s_wave[W];
-  __shared__ Maybe<S> s_prefix;
-
-  const uint32_t epoch = block_epoch(a.ctl, &s_u32[0]);
-  const uint32_t t = block_ticket(a.ctl, &s_u32[1]);
   const int lane = lane_id();
   const int w = threadIdx.x / 64;
-  const int64_t base = (a.tile0 + (int64_t)t) * TILE;
+  const int64_t base = (int64_t)t * TILE;
+  const int64_t last_e = min(base + TILE, a.e_end) - 1;
 
-  // ---- segment range of the tile, staged in LDS when it fits
-  // Segment g in [0, S) covers [start(g), start(g+1)); "segment" S collects
-  // the elements after the last one (start(S+1) = +inf) and elements before
-  // start(0) belong to "segment" -1.  Neither produces a result.
+  // ---- segment range of the tile, staged in LDS when it fits.  Segment g in
+  // [0, S) covers [start(g), start(g+1)); "segment" S collects elements after
+  // the last one (start(S+1) = +inf); elements before start(0) belong to
+  // "segment" -1.  Neither produces a result.
   auto sv = [&](int64_t g) -> int64_t { return g > a.nseg ? INT64_MAX : a.st.start(g); };
-  if (threadIdx.x == 0) {
-    const int64_t lo_e = max(base, a.e_begin);
-    const int64_t hi_e = min(base + TILE, a.e_end) - 1;
-    s_range[0] = max((int64_t)0, seg_search(a.st, 0, a.nseg, lo_e));
-    s_range[1] = max((int64_t)0, seg_search(a.st, 0, a.nseg, hi_e));
-  }
+  if (threadIdx.x == 0) a.st.tile_range(t, base, last_e, a.nseg, &s_range[0], &s_range[1]);
   __syncthreads();
   const int64_t g_lo = s_range[0], g_hi = s_range[1];
   const int64_t cnt = g_hi - g_lo + 2;  // starts g_lo .. g_hi+1
@@ -121,9 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   if (staged)
     for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) s_starts[i] = sv(g_lo + i);
   __syncthreads();
-  auto start_of = [&](int64_t g) -> int64_t {
-    return staged ? s_starts[g - g_lo] : sv(g);
-  };
+  auto start_of = [&](int64_t g) -> int64_t { return staged ? s_starts[g - g_lo] : sv(g); };
   auto search = [&](int64_t e) -> int64_t {
     int64_t x = g_lo, y = g_hi + 1;
     while (x < y) {
@@ -133,23 +145,20 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     return x - 1;  // >= g_lo - 1
   };
 
-  // ---- load and flag
+  // ---- load and flag segment heads
   S x[K][VEC];
   int64_t g0[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
     T v[VEC];
-    if (e0 >= a.e_begin && e0 + VEC <= a.e_end) {
+    if (e0 + VEC <= a.e_end) {
       Vec<T> r = load16_stream(a.in + e0);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v[j] = r.e[j];
     } else {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const int64_t e = e0 + j;
-        v[j] = (e >= a.e_begin && e < a.e_end) ? a.in[e] : T();
-      }
+      for (int j = 0; j < VEC; ++j) v[j] = e0 + j < a.e_end ? a.in[e0 + j] : T();
     }
     int64_t g = search(e0);
     g0[k] = g;
@@ -158,10 +167,9 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
 #pragma unroll
     for (int j = 0; j < VEC; ++j) {
       const int64_t e = e0 + j;
-      const bool valid = e >= a.e_begin && e < a.e_end;
       while (nxt <= e && g < g_hi) { ++g; cur = nxt; nxt = start_of(g + 1); }
       x[k][j].v = v[j];
-      x[k][j].f = (!valid || cur == e) ? 1u : 0u;
+      x[k][j].f = (e >= a.e_end || cur == e) ? 1u : 0u;
     }
 #pragma unroll
     for (int j = 1; j < VEC; ++j) x[k][j] = sop(x[k][j - 1], x[k][j]);
@@ -211,10 +219,10 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
       if (!agg.v.f && lane == 0 && !last)
         Tagged<S>::publish(slot, sop(X.v, agg.v), make_tag(epoch, ST_PREFIX));
     }
-    if (lane == 0) s_prefix = X;
+    if (lane == 0) *s_prefix = X;
   }
   __syncthreads();
-  const Maybe<S> XW = mcombine(sop, s_prefix, wp);
+  const Maybe<S> XW = mcombine(sop, *s_prefix, wp);
 
   // ---- results at segment ends
 #pragma unroll
@@ -226,12 +234,13 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
 #pragma unroll
     for (int j = 0; j < VEC; ++j) {
       const int64_t e = e0 + j;
-      if (e < a.e_begin || e >= a.e_end) continue;
-      while (nxt <= e && g < g_hi) { ++g; nxt = start_of(g + 1); }
-      if (nxt == e + 1 && g >= 0 && g < a.nseg) {
-        T r = P.ok ? sop(P.v, x[k][j]).v : x[k][j].v;
-        if (a.has_seed) r = op(a.seed, r);  // seed on the left
-        a.out[g] = r;
+      if (e < a.e_end) {
+        while (nxt <= e && g < g_hi) { ++g; nxt = start_of(g + 1); }
+        if (nxt == e + 1 && g >= 0 && g < a.nseg) {
+          T r = P.ok ? sop(P.v, x[k][j]).v : x[k][j].v;
+          if (a.has_seed) r = op(a.seed, r);  // seed on the left
+          a.out[g] = r;
+        }
       }
     }
   }
@@ -243,38 +252,77 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     for (int64_t g = s0 + threadIdx.x; g < s1; g += BLOCK)
       if (a.st.start(g) == a.st.start(g + 1)) a.out[g] = a.seed;
   }
-  block_retire(a.ctl, gridDim.x);
+}
+
+// Persistent workgroups taking tiles by ticket (next ticket prefetched).
+template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
+  __shared__ uint64_t s_tick[2];
+  __shared__ int64_t s_range[2];
+  __shared__ int64_t s_starts[CAP];
+  __shared__ Maybe<SegV<T>> s_wave[BLOCK / 64];
+  __shared__ Maybe<SegV<T>> s_prefix;
+  if (threadIdx.x == 0) s_tick[0] = take_ticket(a.ctl);
+  __syncthreads();
+  const uint64_t tk0 = s_tick[0];
+  const uint32_t epoch = (uint32_t)(tk0 >> 32);
+  uint32_t t = (uint32_t)tk0;
+  int cur = 0;
+  while (t < a.ntiles) {
+    if (threadIdx.x == 0) s_tick[cur ^ 1] = take_ticket(a.ctl);
+    seg_tile<T, Op, Starts, BLOCK, K, CAP>(a, t, epoch, s_range, s_starts, s_wave, &s_prefix);
+    __syncthreads();
+    cur ^= 1;
+    t = (uint32_t)s_tick[cur];
+  }
+  block_retire(a.ctl, gridDim.x, epoch);
 }
 
 template <typename T> struct SegTune {
   static constexpr int BLOCK = 256;
-  static constexpr int K = 4;
+  static constexpr int K = 8;  // 32 KiB tiles
   static constexpr int CAP = 1024;
 };
 
 template <typename T, typename Op, typename Starts>
-int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg, int64_t e_begin,
-                    int64_t e_end, const T* seed, bool check_empty) {
+int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg, int64_t e_end,
+                    const T* seed, bool check_empty, const void* offsets, size_t offset_bytes) {
   constexpr int BLOCK = SegTune<T>::BLOCK, K = SegTune<T>::K, CAP = SegTune<T>::CAP;
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
   constexpr int NW = Tagged<SegV<T>>::NW;
   if ((uintptr_t)in % 16) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: input must be 16-byte aligned");
-  if (e_end < e_begin) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: decreasing offsets");
-  const int64_t tile0 = e_begin / TILE;
-  int64_t ntiles = e_end > e_begin ? ceil_div(e_end, TILE) - tile0 : 0;
+  const int64_t ntiles = ceil_div(e_end, TILE);
   if (ntiles == 0) {
     // no elements: every segment is empty
     if (!seed || !check_empty) return ACC_SUCCESS;
     return acc_hip_fill_async(s, out, seed, sizeof(T), (size_t)nseg);
   }
-  if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: too many elements");
+  if (ntiles >= 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: too many elements");
   SyncCtl* ctl;
   uint64_t* status;
-  ACC_CHECK(stream_workspace(s, (size_t)ntiles * NW * 8, &ctl, &status));
-  SegArgs<T, Starts> a{in, out, st, nseg, e_begin, e_end, tile0, (uint32_t)ntiles, seed ? 1u : 0u,
-                       seed ? *seed : T(), check_empty ? 1u : 0u, ctl, status};
-  hipLaunchKernelGGL((k_fold_seg<T, Op, Starts, BLOCK, K, CAP>), dim3((unsigned)ntiles), dim3(BLOCK), 0,
-                     s->stream, a);
+  const size_t status_bytes = ((size_t)ntiles * NW * 8 + 255) / 256 * 256;
+  const size_t tseg_bytes = offsets ? (size_t)(ntiles + 1) * 8 : 0;
+  ACC_CHECK(stream_workspace(s, status_bytes + tseg_bytes, &ctl, &status));
+  if constexpr (!std::is_same<Starts, UniformStarts>::value) {
+    int64_t* tseg = (int64_t*)((char*)status + status_bytes);
+    const unsigned pg = (unsigned)((ntiles + 1 + 3) / 4);
+    if (offset_bytes == 4)
+      hipLaunchKernelGGL(k_tile_seg<int32_t>, dim3(pg), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
+                         TILE, (uint32_t)ntiles, tseg);
+    else
+      hipLaunchKernelGGL(k_tile_seg<int64_t>, dim3(pg), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
+                         TILE, (uint32_t)ntiles, tseg);
+    ACC_HIP_CHECK(hipGetLastError());
+    st.tseg = tseg;
+  }
+  SegArgs<T, Starts> a{in, out, st, nseg, e_end, (uint32_t)ntiles, seed ? 1u : 0u, seed ? *seed : T(),
+                       check_empty ? 1u : 0u, ctl, status};
+  auto kern = k_fold_seg<T, Op, Starts, BLOCK, K, CAP>;
+  const DeviceInfo* di;
+  ACC_CHECK(device_info(s->device, &di));
+  int64_t grid = (int64_t)di->compute_units * blocks_per_cu((const void*)kern, BLOCK);
+  if (grid > ntiles) grid = ntiles;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BLOCK), 0, s->stream, a);
   ACC_HIP_CHECK(hipGetLastError());
   return ACC_SUCCESS;
 }
@@ -285,8 +333,8 @@ int fold_dim(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void
     using T = decltype(z);
     return with_op<T>(op, [&](auto f) {
       using Op = decltype(f);
-      return fold_seg_launch<T, Op, UniformStarts>(s, (const T*)in, (T*)out, UniformStarts{inner}, outer, 0,
-                                                   outer * inner, (const T*)seed, false);
+      return fold_seg_launch<T, Op, UniformStarts>(s, (const T*)in, (T*)out, UniformStarts{inner}, outer,
+                                                   outer * inner, (const T*)seed, false, nullptr, 0);
     });
   });
 }
@@ -317,11 +365,13 @@ extern "C" int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const voi
         T* rout = (T*)out + r * nseg;
         int rc;
         if (I == 4)
-          rc = fold_seg_launch<T, Op, OffsetStarts<int32_t>>(s, rin, rout, OffsetStarts<int32_t>{(const int32_t*)offsets},
-                                                             nseg, 0, inner, (const T*)seed, true);
+          rc = fold_seg_launch<T, Op, OffsetStarts<int32_t>>(
+              s, rin, rout, OffsetStarts<int32_t>{(const int32_t*)offsets, nullptr}, nseg, inner,
+              (const T*)seed, true, offsets, 4);
         else
-          rc = fold_seg_launch<T, Op, OffsetStarts<int64_t>>(s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets},
-                                                             nseg, 0, inner, (const T*)seed, true);
+          rc = fold_seg_launch<T, Op, OffsetStarts<int64_t>>(
+              s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr}, nseg, inner,
+              (const T*)seed, true, offsets, 8);
         if (rc) return rc;
       }
       return (int)ACC_SUCCESS;

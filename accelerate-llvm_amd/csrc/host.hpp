@@ -84,4 +84,7 @@ inline int set_device(int device) {
 
 size_t type_size(acc_type t);
 
+// Resident workgroups per CU of a kernel at a block size (cached).
+int blocks_per_cu(const void* kernel, int block);
+
 }  // namespace acc

@@ -28,9 +28,8 @@
 namespace acc {
 
 struct alignas(64) SyncCtl {
-  uint32_t ticket;  // next tile ticket
-  uint32_t done;    // workgroups finished in this launch
-  uint32_t epoch;   // launch counter (tags)
+  uint64_t tick;  // high 32 bits: launch epoch; low 32 bits: next tile ticket
+  uint32_t done;  // workgroups finished in this launch
   uint32_t pad[13];
 };
 
@@ -78,17 +77,15 @@ template <typename T> struct Tagged {
   }
 };
 
-// Take the next tile ticket (thread 0 of the block) and broadcast it.
-__device__ __forceinline__ uint32_t block_ticket(SyncCtl* ctl, uint32_t* lds_slot) {
-  if (threadIdx.x == 0) *lds_slot = add_relaxed(&ctl->ticket, 1u);
-  __syncthreads();
-  uint32_t t = *lds_slot;
-  __syncthreads();
-  return t;
+// Take a ticket (thread 0) and broadcast it: high 32 bits epoch, low 32 bits
+// the ticket.  One atomic gives both, so a tile costs one round trip here.
+__device__ __forceinline__ uint64_t take_ticket(SyncCtl* ctl) {
+  return __hip_atomic_fetch_add(&ctl->tick, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t block_epoch(SyncCtl* ctl, uint32_t* lds_slot) {
-  if (threadIdx.x == 0) *lds_slot = ld_relaxed(&ctl->epoch);
+  if (threadIdx.x == 0)
+    *lds_slot = (uint32_t)(__hip_atomic_load(&ctl->tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
   __syncthreads();
   uint32_t e = *lds_slot;
   __syncthreads();
@@ -96,15 +93,15 @@ __device__ __forceinline__ uint32_t block_epoch(SyncCtl* ctl, uint32_t* lds_slot
 }
 
 // Called once per workgroup after its last use of the status words.  The last
-// workgroup of the launch resets the ticket and advances the epoch.
-__device__ __forceinline__ void block_retire(SyncCtl* ctl, uint32_t nblocks) {
+// workgroup of the launch resets the ticket and advances the epoch.  Every
+// workgroup has taken all its tickets by then.
+__device__ __forceinline__ void block_retire(SyncCtl* ctl, uint32_t nblocks, uint32_t epoch) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t prev = add_relaxed(&ctl->done, 1u);
     if (prev == nblocks - 1) {
-      st_relaxed(&ctl->ticket, 0u);
       st_relaxed(&ctl->done, 0u);
-      add_relaxed(&ctl->epoch, 1u);
+      __hip_atomic_store(&ctl->tick, (uint64_t)(epoch + 1u) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -114,13 +111,19 @@ __device__ __forceinline__ void block_retire(SyncCtl* ctl, uint32_t nblocks) {
 // `status` holds NW words per tile.  Predecessors publish ST_AGGREGATE (their
 // own reduction) early and ST_PREFIX (inclusive of everything before them)
 // once known; tile 0 publishes ST_PREFIX directly.
-template <typename T, typename Op>
+//
+// The first window covers the 64 nearest predecessors (one per lane); later
+// windows give each lane S consecutive tiles (64*S per round trip), which
+// bounds the latency of a deep look-back without reading far on the common,
+// shallow one.  Lanes whose tile is not yet published re-poll with an
+// exponential back-off.
+template <typename T, typename Op, int S = 4>
 __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch, Op op) {
   constexpr int NW = Tagged<T>::NW;
   const int lane = lane_id();
   const uint32_t tag_a = make_tag(epoch, ST_AGGREGATE);
   const uint32_t tag_p = make_tag(epoch, ST_PREFIX);
-  // segmented in-order combine: a lane flagged PREFIX starts a new run
+  // segmented in-order combine: an element flagged PREFIX starts a new run
   auto seg = [op](Maybe<T> a, Maybe<T> b) {
     Maybe<T> r;
     r.v = b.ok ? b.v : op(a.v, b.v);
@@ -128,31 +131,62 @@ __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch
     return r;
   };
   bool have_acc = false;
-  T acc;
-  int64_t hi = (int64_t)tile - 1;  // highest predecessor covered by this window
+  T acc{};
+  int64_t hi = (int64_t)tile - 1;  // highest predecessor not yet combined
+  int per = 1;                      // tiles per lane in this window
   while (true) {
-    int64_t idx = hi - 63 + lane;
-    T v{};
-    uint32_t tag = 0;
-    if (idx >= 0) {
-      while (true) {
-        tag = Tagged<T>::peek(status + (size_t)idx * NW, v);
-        if (tag == tag_a || tag == tag_p) break;
-        __builtin_amdgcn_s_sleep(1);
+    const int64_t lo = hi - 64 * per + 1;
+    const int64_t first = lo + (int64_t)lane * per;
+    T v[S];
+    bool is_p[S];
+    uint32_t ready = 0;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      v[k] = T{};
+      is_p[k] = false;
+      if (k >= per || first + k < 0) {
+        ready |= 1u << k;
+        is_p[k] = first + k < 0;  // before tile 0: cut off by tile 0's PREFIX
       }
-    } else {
-      tag = tag_p;  // lanes before tile 0 are cut off by tile 0's PREFIX
     }
-    const bool is_p = (tag == tag_p);
+    int backoff = 1;
+    while (true) {
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        if (!(ready >> k & 1u)) {
+          const uint32_t tag = Tagged<T>::peek(status + (size_t)(first + k) * NW, v[k]);
+          if (tag == tag_a || tag == tag_p) {
+            ready |= 1u << k;
+            is_p[k] = tag == tag_p;
+          }
+        }
+      }
+      const bool done = ready == (1u << S) - 1u;
+      if (__all(done)) break;
+      if (backoff == 1) __builtin_amdgcn_s_sleep(1);
+      else if (backoff == 2) __builtin_amdgcn_s_sleep(2);
+      else if (backoff == 4) __builtin_amdgcn_s_sleep(4);
+      else __builtin_amdgcn_s_sleep(8);
+      backoff = backoff < 8 ? backoff * 2 : 8;
+    }
+    // lane-level segmented fold of its tiles, in order
     Maybe<T> m;
-    m.v = v;
-    m.ok = is_p ? 1u : 0u;
-    Maybe<T> s = read_lane(wave_inclusive_scan(m, seg), 63);
-    const uint64_t any_p = __ballot(is_p);
-    // s.v = combination of lanes from the highest PREFIX lane (or lane 0) to 63
+    m.v = v[0];
+    m.ok = is_p[0] ? 1u : 0u;
+#pragma unroll
+    for (int k = 1; k < S; ++k) {
+      if (k < per) {
+        Maybe<T> x;
+        x.v = v[k];
+        x.ok = is_p[k] ? 1u : 0u;
+        m = seg(m, x);
+      }
+    }
+    const Maybe<T> s = read_lane(wave_inclusive_scan(m, seg), 63);
     if (have_acc) acc = op(s.v, acc); else { acc = s.v; have_acc = true; }
-    if (any_p) break;
-    hi -= 64;
+    if (s.ok) break;
+    hi = lo - 1;
+    per = S;
   }
   return acc;
 }

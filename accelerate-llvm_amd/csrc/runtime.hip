@@ -71,6 +71,21 @@ int device_info(int device, const DeviceInfo** out) {
   return ACC_SUCCESS;
 }
 
+int blocks_per_cu(const void* kernel, int block) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 1;
+  }
+  cache[kernel] = nb;
+  return nb;
+}
+
 // ------------------------------------------------------------------- memory
 struct Accounting {
   std::mutex mu;

@@ -195,30 +195,55 @@ __device__ __forceinline__ void scan_tile(const ScanArgs<T>& a, Op op, uint32_t 
   }
 }
 
+// Persistent workgroups: each takes tiles by ticket until none are left,
+// prefetching the next ticket while it scans the current tile.
 template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs<T> a) {
   constexpr int VEC = Vec<T>::N;
   constexpr int TILE = BLOCK * K * VEC;
-  __shared__ uint32_t s_u32[2];
+  __shared__ uint64_t s_tick[2];
   __shared__ Maybe<T> s_wave[BLOCK / 64];
   __shared__ Maybe<T> s_prefix;
   using OpT = typename std::conditional<REVERSE, Flip<Op>, Op>::type;
   OpT op{};
-  const uint32_t epoch = block_epoch(a.ctl, &s_u32[0]);
-  const uint32_t t = block_ticket(a.ctl, &s_u32[1]);
-  const int64_t p = REVERSE ? (int64_t)(a.ntiles - 1 - t) : (int64_t)t;
-  const int64_t base = p * TILE;
-  if (base + TILE <= a.n)
-    scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, false>(a, op, t, base, epoch, s_wave, &s_prefix);
-  else
-    scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, true>(a, op, t, base, epoch, s_wave, &s_prefix);
-  block_retire(a.ctl, gridDim.x);
+  if (threadIdx.x == 0) s_tick[0] = take_ticket(a.ctl);
+  __syncthreads();
+  const uint64_t tk0 = s_tick[0];
+  const uint32_t epoch = (uint32_t)(tk0 >> 32);
+  uint32_t t = (uint32_t)tk0;
+  int cur = 0;
+  while (t < a.ntiles) {
+    if (threadIdx.x == 0) s_tick[cur ^ 1] = take_ticket(a.ctl);
+    const int64_t p = REVERSE ? (int64_t)(a.ntiles - 1 - t) : (int64_t)t;
+    const int64_t base = p * TILE;
+    if (base + TILE <= a.n)
+      scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, false>(a, op, t, base, epoch, s_wave, &s_prefix);
+    else
+      scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, true>(a, op, t, base, epoch, s_wave, &s_prefix);
+    __syncthreads();
+    cur ^= 1;
+    t = (uint32_t)s_tick[cur];
+  }
+  block_retire(a.ctl, gridDim.x, epoch);
 }
 
 template <typename T> struct ScanTune {
   static constexpr int BLOCK = 256;
-  static constexpr int K = 4;  // dwordx4 vectors per thread
+  static constexpr int K = 8;  // dwordx4 vectors per thread: 32 KiB tiles
 };
+
+template <typename T, typename Op, bool REVERSE, bool ALIGNED>
+int scan_launch(acc_stream_t s, const ScanArgs<T>& a) {
+  constexpr int BLOCK = ScanTune<T>::BLOCK, K = ScanTune<T>::K;
+  auto kern = k_scan<T, Op, BLOCK, K, REVERSE, ALIGNED>;
+  const DeviceInfo* di;
+  ACC_CHECK(device_info(s->device, &di));
+  int64_t grid = (int64_t)di->compute_units * blocks_per_cu((const void*)kern, BLOCK);
+  if (grid > (int64_t)a.ntiles) grid = a.ntiles;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BLOCK), 0, s->stream, a);
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
 
 template <typename T, typename Op>
 int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const T* in, int64_t n,
@@ -236,16 +261,8 @@ int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const 
   ACC_CHECK(stream_workspace(s, (size_t)ntiles * NW * 8, &ctl, &status));
   ScanArgs<T> a{in, out, n, (uint32_t)ntiles, exclusive ? 1u : 0u, seed ? 1u : 0u, seed ? *seed : T(),
                 total_out, seed_out, ctl, status};
-  dim3 g((unsigned)ntiles), b(BLOCK);
-  if (reverse) {
-    if (aligned) hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, true, true>), g, b, 0, s->stream, a);
-    else hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, true, false>), g, b, 0, s->stream, a);
-  } else {
-    if (aligned) hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, false, true>), g, b, 0, s->stream, a);
-    else hipLaunchKernelGGL((k_scan<T, Op, BLOCK, K, false, false>), g, b, 0, s->stream, a);
-  }
-  ACC_HIP_CHECK(hipGetLastError());
-  return ACC_SUCCESS;
+  if (reverse) return aligned ? scan_launch<T, Op, true, true>(s, a) : scan_launch<T, Op, true, false>(s, a);
+  return aligned ? scan_launch<T, Op, false, true>(s, a) : scan_launch<T, Op, false, false>(s, a);
 }
 
 // rank >= 2 scans: one call per row for now (each row is an independent scan)

@@ -32,7 +32,7 @@ template <typename T> struct StArgs {
   const T* in;
   T* out;
   int64_t h, w;
-  int64_t tiles_x, tiles_y;
+  uint32_t tiles_x, tiles_y;
   int boundary;
   T cval;
   T wt[81];  // (2RY+1) x (2RX+1) weights, row-major
@@ -60,36 +60,28 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   __shared__ __align__(16) T tile[LH * LW];
 
   // XCD-aware tile order: consecutive workgroups land on different XCDs, so
-  // give each XCD a contiguous band of tiles (vertical neighbours share halo
-  // rows in that XCD's L2).
-  const int64_t nt = a.tiles_x * a.tiles_y;
-  int64_t b = blockIdx.x;
-  {
-    const int64_t per = (nt + 7) / 8;
-    const int64_t xcd = b % 8, k = b / 8;
-    const int64_t bb = xcd * per + k;
-    if (nt % 8 == 0) b = bb;
-  }
-  const int64_t tx = b / a.tiles_y, ty = b % a.tiles_y;  // column-major: walk down a column band
-  const int64_t x0 = tx * ST_TW, y0 = ty * ST_TH;
+  // when the tile count allows, each XCD gets a contiguous band of tiles
+  // (vertical neighbours then share halo rows in that XCD's L2).
+  const uint32_t nt = a.tiles_x * a.tiles_y;
+  uint32_t b = blockIdx.x;
+  if (nt % 8u == 0u) b = (b % 8u) * (nt / 8u) + b / 8u;
+  const uint32_t tx = b / a.tiles_y;  // column-major: walk down a column band
+  const uint32_t ty = b - tx * a.tiles_y;
+  const int64_t x0 = (int64_t)tx * ST_TW, y0 = (int64_t)ty * ST_TH;
 
-  // ---- stage the halo tile
-  const bool interior = y0 - RY >= 0 && y0 + ST_TH + RY <= a.h && x0 - RX >= 0 && x0 + ST_TW + RX <= a.w;
-  if (interior) {
-    for (int i = threadIdx.x; i < LH * LW; i += ST_BLOCK) {
-      const int r = i / LW, c = i - r * LW;
-      tile[i] = a.in[(y0 - RY + r) * a.w + (x0 - RX + c)];
+  // ---- stage the halo tile; coordinates outside the array go through the
+  // boundary map (constant: the value itself)
+  const bool constant = a.boundary == ACC_BOUNDARY_CONSTANT;
+  for (int i = threadIdx.x; i < LH * LW; i += ST_BLOCK) {
+    const int r = i / LW, c = i - r * LW;
+    int64_t gy = y0 - RY + r, gx = x0 - RX + c;
+    const bool inside = (uint64_t)gy < (uint64_t)a.h && (uint64_t)gx < (uint64_t)a.w;
+    if (!inside) {
+      gy = bound_index(a.boundary, gy, a.h);
+      gx = bound_index(a.boundary, gx, a.w);
     }
-  } else {
-    for (int i = threadIdx.x; i < LH * LW; i += ST_BLOCK) {
-      const int r = i / LW, c = i - r * LW;
-      const int64_t gy = y0 - RY + r, gx = x0 - RX + c;
-      T v;
-      if (gy >= 0 && gy < a.h && gx >= 0 && gx < a.w) v = a.in[gy * a.w + gx];
-      else if (a.boundary == ACC_BOUNDARY_CONSTANT) v = a.cval;
-      else v = a.in[bound_index(a.boundary, gy, a.h) * a.w + bound_index(a.boundary, gx, a.w)];
-      tile[i] = v;
-    }
+    T v = a.in[gy * a.w + gx];
+    tile[i] = (inside || !constant) ? v : a.cval;
   }
   __syncthreads();
 
@@ -126,9 +118,7 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
     const int64_t gy = y0 + cy + oy;
     const int64_t gx = x0 + cx;
     T* o = a.out + gy * a.w + gx;
-    if (gy >= a.h) {
-    } else
-    if (gx + ST_PX <= a.w && ((uintptr_t)o % 16) == 0 && ST_PX * sizeof(T) % 16 == 0) {
+    if (gy < a.h && gx + ST_PX <= a.w && ((uintptr_t)o % 16) == 0 && ST_PX * sizeof(T) % 16 == 0) {
 #pragma unroll
       for (int q = 0; q < (int)(ST_PX * sizeof(T) / 16); ++q) {
         Vec<T> r;
@@ -136,7 +126,7 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
         for (int j = 0; j < Vec<T>::N; ++j) r.e[j] = acc[oy][q * Vec<T>::N + j];
         store16(o + q * Vec<T>::N, r);
       }
-    } else {
+    } else if (gy < a.h) {
 #pragma unroll
       for (int ox = 0; ox < ST_PX; ++ox)
         if (gx + ox < a.w) o[ox] = acc[oy][ox];
@@ -152,14 +142,15 @@ int stencil_launch(acc_stream_t s, const T* w, int boundary, const T* cval, cons
   a.out = out;
   a.h = h;
   a.w = wd;
-  a.tiles_x = ceil_div(wd, ST_TW);
-  a.tiles_y = ceil_div(h, ST_TH);
+  const int64_t tiles_x = ceil_div(wd, ST_TW), tiles_y = ceil_div(h, ST_TH);
+  if (tiles_x * tiles_y > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "stencil: array too large");
+  a.tiles_x = (uint32_t)tiles_x;
+  a.tiles_y = (uint32_t)tiles_y;
   a.boundary = boundary;
   a.cval = cval ? *cval : T();
   const int taps = (2 * RY + 1) * (2 * RX + 1);
   for (int i = 0; i < 81; ++i) a.wt[i] = i < taps ? w[i] : T();
-  const int64_t nt = a.tiles_x * a.tiles_y;
-  if (nt > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "stencil: array too large");
+  const int64_t nt = tiles_x * tiles_y;
   hipLaunchKernelGGL((k_stencil2d<T, RY, RX>), dim3((unsigned)nt), dim3(ST_BLOCK), 0, s->stream, a);
   ACC_HIP_CHECK(hipGetLastError());
   return ACC_SUCCESS;

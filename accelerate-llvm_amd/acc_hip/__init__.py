@@ -71,6 +71,21 @@ class DeviceProps(ctypes.Structure):
     ]
 
 
+class BatchProblem(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("type", ctypes.c_int32),
+        ("op", ctypes.c_int32),
+        ("has_seed", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+        ("in_", c_void_p),
+        ("out", c_void_p),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+BATCH_SCAN, BATCH_FOLD = 0, 1
+
 # exported symbol -> (restype, argtypes); every declaration in the header
 SIGNATURES = {
     "acc_hip_last_error": (c_char_p, []),
@@ -122,6 +137,10 @@ SIGNATURES = {
                                 c_void_p, c_int64]),
     "acc_hip_stencil2d": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int64,
                                   c_int64, c_void_p]),
+    "acc_hip_batch_create": (c_int, [c_int, POINTER(BatchProblem), c_int64, c_int, POINTER(c_void_p)]),
+    "acc_hip_batch_launch": (c_int, [c_void_p, c_void_p]),
+    "acc_hip_batch_tile": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_uint32)]),
+    "acc_hip_batch_destroy": (c_int, [c_void_p]),
 }
 
 
@@ -328,3 +347,81 @@ def stencil2d(stream, ry, rx, weights, boundary, constant, src, height, width, o
     keep, cp = _scalar(dt, constant)
     check(lib().acc_hip_stencil2d(stream.handle, acc_type(dt), ry, rx, w.ctypes.data_as(c_void_p), boundary, cp,
                                   _p(src), height, width, _p(out)))
+
+
+def _bits(dtype, v):
+    a = np.zeros(1, np.uint64)
+    b = np.array([v], dtype=dtype)
+    a.view(np.uint8)[: b.nbytes] = b.view(np.uint8)
+    return int(a[0])
+
+
+class Batch:
+    """Many scans/folds in one persistent launch (acc_hip_batch_*)."""
+
+    def __init__(self, device, problems, tile=0):
+        """problems: list of (kind, op, seed_or_None, src DeviceArray/View, n, out DeviceArray/View)."""
+        arr = (BatchProblem * len(problems))()
+        for i, (kind, op, seed, src, n, out) in enumerate(problems):
+            arr[i].kind = kind
+            arr[i].type = acc_type(src.dtype)
+            arr[i].op = op
+            arr[i].has_seed = 0 if seed is None else 1
+            arr[i].n = n
+            arr[i].in_ = src.ptr
+            arr[i].out = out.ptr
+            arr[i].seed = 0 if seed is None else _bits(src.dtype, seed)
+        h = c_void_p()
+        check(lib().acc_hip_batch_create(device, arr, len(problems), tile, ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self, stream):
+        check(lib().acc_hip_batch_launch(self.handle, stream.handle))
+
+    def tile(self):
+        k, t = c_int(), ctypes.c_uint32()
+        check(lib().acc_hip_batch_tile(self.handle, ctypes.byref(k), ctypes.byref(t)))
+        return k.value, t.value
+
+    def destroy(self):
+        if self.handle:
+            lib().acc_hip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured hipGraph (acc_hip_graph_*)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def capture(cls, stream, fn):
+        check(lib().acc_hip_graph_begin_capture(stream.handle))
+        try:
+            fn()
+        finally:
+            h = c_void_p()
+            rc = lib().acc_hip_graph_end_capture(stream.handle, ctypes.byref(h))
+        check(rc)
+        return cls(h)
+
+    def launch(self, stream):
+        check(lib().acc_hip_graph_launch(self.handle, stream.handle))
+
+    def destroy(self):
+        if self.handle:
+            lib().acc_hip_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

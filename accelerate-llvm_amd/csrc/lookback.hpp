@@ -107,7 +107,10 @@ __device__ __forceinline__ void block_retire(SyncCtl* ctl, uint32_t nblocks, uin
 }
 
 // Decoupled look-back, executed by one full wave of the block that owns tile
-// `tile` (> 0).  Returns the in-order combination of all tiles before `tile`.
+// `tile` (> first).  Returns the in-order combination of tiles [first, tile).
+// Several independent problems can share one status array: tile `first`
+// (the problem's first tile) always publishes ST_PREFIX and everything below
+// it is cut off.
 // `status` holds NW words per tile.  Predecessors publish ST_AGGREGATE (their
 // own reduction) early and ST_PREFIX (inclusive of everything before them)
 // once known; tile 0 publishes ST_PREFIX directly.
@@ -118,7 +121,7 @@ __device__ __forceinline__ void block_retire(SyncCtl* ctl, uint32_t nblocks, uin
 // shallow one.  Lanes whose tile is not yet published re-poll with an
 // exponential back-off.
 template <typename T, typename Op, int S = 4>
-__device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch, Op op) {
+__device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch, Op op, uint32_t first = 0) {
   constexpr int NW = Tagged<T>::NW;
   const int lane = lane_id();
   const uint32_t tag_a = make_tag(epoch, ST_AGGREGATE);
@@ -136,7 +139,7 @@ __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch
   int per = 1;                      // tiles per lane in this window
   while (true) {
     const int64_t lo = hi - 64 * per + 1;
-    const int64_t first = lo + (int64_t)lane * per;
+    const int64_t idx0 = lo + (int64_t)lane * per;
     T v[S];
     bool is_p[S];
     uint32_t ready = 0;
@@ -144,9 +147,9 @@ __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch
     for (int k = 0; k < S; ++k) {
       v[k] = T{};
       is_p[k] = false;
-      if (k >= per || first + k < 0) {
+      if (k >= per || idx0 + k < (int64_t)first) {
         ready |= 1u << k;
-        is_p[k] = first + k < 0;  // before tile 0: cut off by tile 0's PREFIX
+        is_p[k] = idx0 + k < (int64_t)first;  // before the first tile: cut off
       }
     }
     int backoff = 1;
@@ -154,7 +157,7 @@ __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch
 #pragma unroll
       for (int k = 0; k < S; ++k) {
         if (!(ready >> k & 1u)) {
-          const uint32_t tag = Tagged<T>::peek(status + (size_t)(first + k) * NW, v[k]);
+          const uint32_t tag = Tagged<T>::peek(status + (size_t)(idx0 + k) * NW, v[k]);
           if (tag == tag_a || tag == tag_p) {
             ready |= 1u << k;
             is_p[k] = tag == tag_p;

@@ -243,6 +243,41 @@ int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, const void* we
                       acc_boundary boundary, const void* constant, const void* in,
                       int64_t height, int64_t width, void* out);
 
+/* ================================================================ batched
+ * Many independent scans and folds of uneven sizes in ONE persistent launch
+ * (no counterpart in the reference, whose per-call path is
+ * PTX/Execute.hs:835-881).  A batch is built once from problem descriptors
+ * (device pointers; inputs 16-byte aligned, scan outputs too) and can be
+ * launched many times, directly or inside a captured graph.  One launch of a
+ * batch may be in flight at a time.  Problem semantics:
+ *   ACC_BATCH_SCAN: has_seed == 0 -> scanl1 (n outputs);
+ *                   has_seed != 0 -> scanl seed (n + 1 outputs, total last)
+ *   ACC_BATCH_FOLD: out[0] = seed (+) fold of in (fold1 when has_seed == 0)
+ * Element types INT32, INT64, FLOAT32, FLOAT64; combine ADD or MAX. */
+typedef enum acc_batch_kind { ACC_BATCH_SCAN = 0, ACC_BATCH_FOLD = 1 } acc_batch_kind;
+
+typedef struct acc_batch_problem {
+  int32_t kind;     /* acc_batch_kind */
+  int32_t type;     /* acc_type */
+  int32_t op;       /* acc_op */
+  int32_t has_seed;
+  int64_t n;        /* elements (> 0) */
+  const void* in;
+  void* out;
+  uint64_t seed;    /* bit pattern of the seed in the low bytes */
+} acc_batch_problem;
+
+typedef struct acc_batch_s* acc_batch_t;
+
+/* tile = vectors per thread (1, 2 or 4); 0 lets the autotuner time the
+ * candidates on this batch and remember the winner (file named by
+ * $ACC_HIP_TUNE_CACHE, else ~/.cache/accelerate-hip-tune.txt). */
+int acc_hip_batch_create(int device, const acc_batch_problem* problems, int64_t count, int tile,
+                         acc_batch_t* b);
+int acc_hip_batch_launch(acc_batch_t b, acc_stream_t s);
+int acc_hip_batch_tile(acc_batch_t b, int* tile, uint32_t* total_tiles);
+int acc_hip_batch_destroy(acc_batch_t b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,46 @@ class BatchProblem(ctypes.Structure):
 
 BATCH_SCAN, BATCH_FOLD = 0, 1
 
+REQ_FOLD, REQ_SCAN, REQ_FOLD_SEG, REQ_PERMUTE, REQ_STENCIL2D = range(5)
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("skeleton", ctypes.c_int32),
+        ("type", ctypes.c_int32),
+        ("op", ctypes.c_int32),
+        ("has_seed", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("fill", ctypes.c_uint64),
+        ("in_", c_void_p),
+        ("n", ctypes.c_int64),
+        ("m", ctypes.c_int64),
+        ("aux", c_void_p),
+        ("aux_type", ctypes.c_int32),
+        ("direction", ctypes.c_int32),
+        ("ry", ctypes.c_int32),
+        ("rx", ctypes.c_int32),
+        ("boundary", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("weights", c_void_p),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("p50_us", ctypes.c_double),
+        ("p99_us", ctypes.c_double),
+        ("mean_us", ctypes.c_double),
+        ("max_us", ctypes.c_double),
+        ("req_per_s", ctypes.c_double),
+        ("bytes_per_s", ctypes.c_double),
+        ("span_s", ctypes.c_double),
+        ("arena_bytes", c_size_t),
+        ("arena_used", c_size_t),
+    ]
+
 # exported symbol -> (restype, argtypes); every declaration in the header
 SIGNATURES = {
     "acc_hip_last_error": (c_char_p, []),
@@ -141,6 +181,14 @@ SIGNATURES = {
     "acc_hip_batch_launch": (c_int, [c_void_p, c_void_p]),
     "acc_hip_batch_tile": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_uint32)]),
     "acc_hip_batch_destroy": (c_int, [c_void_p]),
+    "acc_hip_server_create": (c_int, [c_int, c_int, c_size_t, POINTER(c_void_p)]),
+    "acc_hip_server_submit": (c_int, [c_void_p, POINTER(Request), c_int64, c_int, POINTER(ctypes.c_uint64)]),
+    "acc_hip_server_wait": (c_int, [c_void_p, ctypes.c_uint64, POINTER(c_void_p), POINTER(ctypes.c_double)]),
+    "acc_hip_server_release": (c_int, [c_void_p, ctypes.c_uint64]),
+    "acc_hip_server_drain": (c_int, [c_void_p]),
+    "acc_hip_server_stats": (c_int, [c_void_p, POINTER(ServerStats)]),
+    "acc_hip_server_reset_stats": (c_int, [c_void_p]),
+    "acc_hip_server_destroy": (c_int, [c_void_p]),
 }
 
 
@@ -425,3 +473,81 @@ class Graph:
             self.destroy()
         except Exception:
             pass
+
+
+class Server:
+    """The serving loop (acc_hip_server_*)."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=0):
+        h = c_void_p()
+        check(lib().acc_hip_server_create(device, slots, hbm_budget, ctypes.byref(h)))
+        self.handle = h
+        self._keep = []  # host buffers referenced by queued requests (stencil weights)
+
+    @staticmethod
+    def request(skeleton, dtype, op=ADD, seed=None, src=None, n=0, m=0, aux=None, fill=0, direction=0,
+                ry=0, rx=0, boundary=0, weights=None):
+        r = Request()
+        r.skeleton = skeleton
+        r.type = acc_type(dtype)
+        r.op = op
+        r.has_seed = 0 if seed is None else 1
+        r.seed = 0 if seed is None else _bits(dtype, seed)
+        r.fill = _bits(dtype, fill)
+        r.in_ = src.ptr if src is not None else None
+        r.n = n
+        r.m = m
+        r.aux = aux.ptr if aux is not None else None
+        r.aux_type = acc_type(aux.dtype) if aux is not None else INT32
+        r.direction = direction
+        r.ry, r.rx, r.boundary = ry, rx, boundary
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(np.asarray(weights, dtype=dtype).reshape(-1))
+            r.weights = w.ctypes.data_as(c_void_p)
+        return r, w
+
+    def submit(self, req, arrival_offset_ns=0, keep=False):
+        r, w = req
+        rid = ctypes.c_uint64()
+        check(lib().acc_hip_server_submit(self.handle, ctypes.byref(r), int(arrival_offset_ns), 1 if keep else 0,
+                                          ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        p, lat = c_void_p(), ctypes.c_double()
+        check(lib().acc_hip_server_wait(self.handle, rid, ctypes.byref(p), ctypes.byref(lat)))
+        return p.value, lat.value
+
+    def release(self, rid):
+        check(lib().acc_hip_server_release(self.handle, rid))
+
+    def drain(self):
+        check(lib().acc_hip_server_drain(self.handle))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().acc_hip_server_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def reset_stats(self):
+        check(lib().acc_hip_server_reset_stats(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            lib().acc_hip_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def copy_to_host(stream, ptr, count, dtype):
+    out = np.empty(count, dtype)
+    if count:
+        check(lib().acc_hip_memcpy_d2h_async(stream.handle, out.ctypes.data_as(c_void_p), c_void_p(ptr), out.nbytes))
+        stream.synchronize()
+    return out

@@ -1,0 +1,410 @@
+// Serving loop: requests (one skeleton invocation each, inputs resident on
+// the device) arrive over time; a scheduler thread admits them as execution
+// slots and device memory free up, runs admitted requests concurrently on
+// per-slot streams, and records each request's latency from arrival to its
+// result being complete on the device.
+//
+// This is continuous batching in the sense of the serving literature: there
+// is no fixed batch boundary; a request that arrives while others execute is
+// admitted as soon as a slot and its output memory are available, and its
+// kernels overlap on the GPU with those already running.  Admission is by
+// free HBM: outputs are carved from a device arena sized at creation from
+// hipMemGetInfo (or the caller's budget, whichever is smaller); a request
+// whose output does not fit waits, and later small requests may pass it.
+//
+// There is no counterpart in the reference (accelerate-llvm runs one program
+// per `run` call, PTX.hs:140-180); each request executes through the same
+// entry points as the plain calls, so "alone" and "in the mix" results come
+// from the same kernels.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+#include <string.h>
+#include "host.hpp"
+
+namespace acc {
+
+using clk = std::chrono::steady_clock;
+
+// First-fit sub-allocator over one device allocation (offsets, 256-byte units).
+class Arena {
+ public:
+  void init(size_t bytes) {
+    cap_ = bytes;
+    free_.clear();
+    if (bytes) free_[0] = bytes;
+    used_ = 0;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes;
+        const size_t at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t capacity() const { return cap_; }
+  size_t used() const { return used_; }
+
+ private:
+  size_t cap_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;
+};
+
+struct Req {
+  uint64_t id;
+  acc_request r;
+  std::vector<uint8_t> weights;  // stencil weights (host copy)
+  clk::time_point arrival;
+  clk::time_point done;
+  size_t out_bytes = 0, out_off = 0;
+  int slot = -1;
+  int status = ACC_SUCCESS;
+  bool finished = false;
+  bool keep = false;
+};
+
+}  // namespace acc
+
+using namespace acc;
+
+struct acc_server_s {
+  int device = 0;
+  void* arena_base = nullptr;
+  Arena arena;
+  std::vector<acc_stream_t> slots;
+  std::vector<hipEvent_t> slot_event;
+  std::vector<uint64_t> slot_req;  // 0 = idle
+  std::mutex mu;
+  std::condition_variable cv;
+  std::map<uint64_t, std::unique_ptr<Req>> reqs;
+  std::deque<uint64_t> pending;  // arrival order (by arrival time)
+  std::vector<double> latencies_us;
+  clk::time_point first_arrival{}, last_done{};
+  bool any = false;
+  uint64_t next_id = 1;
+  uint64_t completed = 0, failed = 0;
+  double bytes_done = 0;
+  bool stop = false;
+  std::thread worker;
+};
+
+namespace acc {
+
+static size_t request_out_bytes(const acc_request& r) {
+  const size_t B = type_size((acc_type)r.type);
+  switch (r.skeleton) {
+    case ACC_REQ_FOLD: return B;
+    case ACC_REQ_SCAN: return B * (size_t)(r.has_seed ? r.n + 1 : r.n);
+    case ACC_REQ_FOLD_SEG: return B * (size_t)r.m;
+    case ACC_REQ_PERMUTE: return B * (size_t)r.m;
+    case ACC_REQ_STENCIL2D: return B * (size_t)r.n * (size_t)r.m;
+  }
+  return 0;
+}
+
+// algorithmic HBM bytes of a request (for the aggregate bytes/s figure)
+static double request_bytes(const acc_request& r) {
+  const double B = (double)type_size((acc_type)r.type);
+  const double I = r.aux_type == ACC_INT64 ? 8.0 : 4.0;
+  switch (r.skeleton) {
+    case ACC_REQ_FOLD: return r.n * B + B;
+    case ACC_REQ_SCAN: return r.n * B + (r.has_seed ? r.n + 1 : r.n) * B;
+    case ACC_REQ_FOLD_SEG: return r.n * B + (r.m + 1) * I + r.m * B;
+    case ACC_REQ_PERMUTE: return r.n * I + (r.in ? r.n * B : 0) + 2 * r.m * B;
+    case ACC_REQ_STENCIL2D: return 2.0 * r.n * r.m * B;
+  }
+  return 0;
+}
+
+static int run_request(acc_server_t s, Req& q, acc_stream_t st, void* out) {
+  const acc_request& r = q.r;
+  const void* seed = r.has_seed ? (const void*)&r.seed : nullptr;
+  switch (r.skeleton) {
+    case ACC_REQ_FOLD:
+      return acc_hip_fold(st, (acc_type)r.type, (acc_op)r.op, seed, r.in, 1, r.n, out);
+    case ACC_REQ_SCAN:
+      return acc_hip_scan(st, (acc_type)r.type, (acc_op)r.op, (acc_direction)r.direction, seed, r.in, 1, r.n, out);
+    case ACC_REQ_FOLD_SEG:
+      return acc_hip_fold_seg(st, (acc_type)r.type, (acc_op)r.op, seed, r.in, 1, r.n, (acc_type)r.aux_type, r.aux,
+                              r.m, out);
+    case ACC_REQ_PERMUTE: {
+      // defaults: the seed value (0 for a histogram), updated in place
+      uint64_t z = r.seed;
+      int rc = acc_hip_fill_async(st, out, &z, type_size((acc_type)r.type), (size_t)r.m);
+      if (rc) return rc;
+      return acc_hip_permute(st, (acc_type)r.type, (acc_op)r.op, out, out, r.m, r.in, &r.fill,
+                             (acc_type)r.aux_type, r.aux, r.n);
+    }
+    case ACC_REQ_STENCIL2D:
+      return acc_hip_stencil2d(st, (acc_type)r.type, r.ry, r.rx, q.weights.data(), (acc_boundary)r.boundary,
+                               &r.fill, r.in, r.n, r.m, out);
+  }
+  return fail(ACC_ERROR_UNSUPPORTED, "server: request kind %d", r.skeleton);
+}
+
+static void server_loop(acc_server_t s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    const auto now = clk::now();
+    // completions
+    for (size_t k = 0; k < s->slots.size(); ++k) {
+      const uint64_t id = s->slot_req[k];
+      if (!id) continue;
+      if (hipEventQuery(s->slot_event[k]) != hipSuccess) continue;
+      Req& q = *s->reqs[id];
+      q.done = clk::now();
+      q.finished = true;
+      s->slot_req[k] = 0;
+      s->completed++;
+      s->latencies_us.push_back(std::chrono::duration<double, std::micro>(q.done - q.arrival).count());
+      s->bytes_done += request_bytes(q.r);
+      if (!s->any || q.done > s->last_done) s->last_done = q.done;
+      if (!q.keep) {
+        s->arena.release(q.out_off, q.out_bytes);
+        s->reqs.erase(id);
+      }
+      s->cv.notify_all();
+    }
+    // admissions: requests that have arrived, in arrival order; one that does
+    // not fit in the free memory is passed by later ones that do.  The scan
+    // restarts after each admission (the lock is dropped while launching).
+    bool admitted = true;
+    while (admitted) {
+      admitted = false;
+      int slot = -1;
+      for (size_t k = 0; k < s->slots.size(); ++k)
+        if (!s->slot_req[k]) { slot = (int)k; break; }
+      if (slot < 0) break;
+      for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+        Req& q = *s->reqs[*it];
+        if (q.arrival > now) break;  // pending is ordered by arrival
+        size_t off;
+        if (!s->arena.alloc(q.out_bytes, &off)) continue;
+        s->pending.erase(it);
+        q.out_off = off;
+        q.slot = slot;
+        s->slot_req[slot] = q.id;  // reserve the slot while launching
+        void* out = (char*)s->arena_base + off;
+        acc_stream_t st = s->slots[slot];
+        lk.unlock();
+        int rc = run_request(s, q, st, out);
+        if (!rc && hipEventRecord(s->slot_event[slot], st->stream) != hipSuccess) rc = ACC_ERROR_HIP;
+        lk.lock();
+        if (rc) {
+          s->slot_req[slot] = 0;
+          q.status = rc;
+          q.finished = true;
+          q.done = clk::now();
+          s->failed++;
+          s->arena.release(off, q.out_bytes);
+          s->cv.notify_all();
+        }
+        admitted = true;
+        break;
+      }
+    }
+    bool busy = false;
+    for (auto id : s->slot_req) busy |= id != 0;
+    if (s->stop && !busy && s->pending.empty()) break;
+    if (!busy && s->pending.empty()) {
+      s->cv.wait_for(lk, std::chrono::milliseconds(5));
+    } else if (!busy && !s->pending.empty() && s->reqs[s->pending.front()]->arrival > clk::now()) {
+      s->cv.wait_until(lk, s->reqs[s->pending.front()]->arrival);
+    } else {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(2));
+      lk.lock();
+    }
+  }
+}
+
+}  // namespace acc
+
+extern "C" {
+
+int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t* out) {
+  if (!out || slots < 1 || slots > 64) return fail(ACC_ERROR_INVALID_VALUE, "server: 1..64 slots");
+  ACC_CHECK(set_device(device));
+  size_t fr = 0, tot = 0;
+  ACC_CHECK(acc_hip_mem_get_info(device, &fr, &tot));
+  size_t cap = hbm_budget ? std::min(hbm_budget, fr) : fr / 2;
+  cap = cap / 256 * 256;
+  auto s = std::make_unique<acc_server_s>();
+  s->device = device;
+  if (cap) ACC_CHECK(dev_malloc(device, cap, &s->arena_base));
+  s->arena.init(cap);
+  for (int k = 0; k < slots; ++k) {
+    acc_stream_t st;
+    int rc = acc_hip_stream_create(device, &st);
+    if (rc) return rc;
+    s->slots.push_back(st);
+    hipEvent_t e;
+    ACC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    s->slot_event.push_back(e);
+    s->slot_req.push_back(0);
+  }
+  acc_server_s* raw = s.release();
+  raw->worker = std::thread(server_loop, raw);
+  *out = raw;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_submit(acc_server_t s, const acc_request* r, int64_t arrival_offset_ns, int keep_output,
+                          uint64_t* id) {
+  if (!s || !r || !id) return fail(ACC_ERROR_INVALID_VALUE, "server: null argument");
+  if (r->skeleton < ACC_REQ_FOLD || r->skeleton > ACC_REQ_STENCIL2D)
+    return fail(ACC_ERROR_UNSUPPORTED, "server: request kind %d", r->skeleton);
+  if (!type_size((acc_type)r->type)) return fail(ACC_ERROR_UNSUPPORTED, "server: element type %d", r->type);
+  auto q = std::make_unique<Req>();
+  q->r = *r;
+  q->keep = keep_output != 0;
+  q->out_bytes = request_out_bytes(*r);
+  if (q->out_bytes > s->arena.capacity())
+    return fail(ACC_ERROR_OUT_OF_MEMORY, "server: output of %zu bytes exceeds the arena (%zu)", q->out_bytes,
+                s->arena.capacity());
+  if (r->skeleton == ACC_REQ_STENCIL2D) {
+    if (!r->weights) return fail(ACC_ERROR_INVALID_VALUE, "server: stencil weights");
+    const size_t nb = (size_t)(2 * r->ry + 1) * (2 * r->rx + 1) * type_size((acc_type)r->type);
+    q->weights.assign((const uint8_t*)r->weights, (const uint8_t*)r->weights + nb);
+  }
+  q->arrival = clk::now() + std::chrono::nanoseconds(arrival_offset_ns > 0 ? arrival_offset_ns : 0);
+  std::lock_guard<std::mutex> lk(s->mu);
+  q->id = s->next_id++;
+  *id = q->id;
+  if (!s->any || q->arrival < s->first_arrival) s->first_arrival = q->arrival;
+  s->any = true;
+  // keep pending ordered by arrival time
+  auto pos = s->pending.end();
+  while (pos != s->pending.begin() && s->reqs[*std::prev(pos)]->arrival > q->arrival) --pos;
+  s->pending.insert(pos, q->id);
+  s->reqs[q->id] = std::move(q);
+  s->cv.notify_all();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_wait(acc_server_t s, uint64_t id, void** out, double* latency_us) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "server: null");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(ACC_ERROR_INVALID_VALUE, "server: unknown or released request %llu",
+                                       (unsigned long long)id);
+  Req* q = it->second.get();
+  s->cv.wait(lk, [&] { return q->finished; });
+  if (q->status) return fail(q->status, "server: request %llu failed", (unsigned long long)id);
+  if (out) *out = (char*)s->arena_base + q->out_off;
+  if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(q->done - q->arrival).count();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_release(acc_server_t s, uint64_t id) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "server: null");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end() || !it->second->finished)
+    return fail(ACC_ERROR_INVALID_VALUE, "server: request %llu not finished", (unsigned long long)id);
+  if (it->second->status == ACC_SUCCESS) s->arena.release(it->second->out_off, it->second->out_bytes);
+  s->reqs.erase(it);
+  s->cv.notify_all();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_drain(acc_server_t s) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "server: null");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv.wait(lk, [&] {
+    bool busy = false;
+    for (auto id : s->slot_req) busy |= id != 0;
+    return !busy && s->pending.empty();
+  });
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_stats(acc_server_t s, acc_server_stats* st) {
+  if (!s || !st) return fail(ACC_ERROR_INVALID_VALUE, "server: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  memset(st, 0, sizeof *st);
+  st->completed = s->completed;
+  st->failed = s->failed;
+  std::vector<double> l = s->latencies_us;
+  if (!l.empty()) {
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) { return l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
+    st->p50_us = pct(0.50);
+    st->p99_us = pct(0.99);
+    st->max_us = l.back();
+    double sum = 0;
+    for (double v : l) sum += v;
+    st->mean_us = sum / l.size();
+    const double span = std::chrono::duration<double>(s->last_done - s->first_arrival).count();
+    st->span_s = span;
+    if (span > 0) {
+      st->req_per_s = l.size() / span;
+      st->bytes_per_s = s->bytes_done / span;
+    }
+  }
+  st->arena_bytes = s->arena.capacity();
+  st->arena_used = s->arena.used();
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_reset_stats(acc_server_t s) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "server: null");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->latencies_us.clear();
+  s->completed = s->failed = 0;
+  s->bytes_done = 0;
+  s->any = false;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_destroy(acc_server_t s) {
+  if (!s) return ACC_SUCCESS;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  for (auto st : s->slots) acc_hip_stream_destroy(st);
+  for (auto e : s->slot_event) (void)hipEventDestroy(e);
+  if (s->arena_base) dev_free(s->device, s->arena_base);
+  delete s;
+  return ACC_SUCCESS;
+}
+
+}  // extern "C"

@@ -278,6 +278,65 @@ int acc_hip_batch_launch(acc_batch_t b, acc_stream_t s);
 int acc_hip_batch_tile(acc_batch_t b, int* tile, uint32_t* total_tiles);
 int acc_hip_batch_destroy(acc_batch_t b);
 
+/* ================================================================ serving
+ * A request loop (no counterpart in the reference, which runs one program per
+ * `run`, PTX.hs:140-180): requests are skeleton invocations on device-resident
+ * inputs; a scheduler thread admits them as execution slots (streams) and
+ * output memory (a device arena, admission by free HBM) become free, so work
+ * of requests that arrive while others execute overlaps on the GPU.  Latency
+ * is measured from a request's arrival to its result being complete on the
+ * device.  Requests may be submitted ahead with a future arrival time (an
+ * open-loop arrival trace). */
+typedef enum acc_request_kind {
+  ACC_REQ_FOLD = 0,      /* fold/fold1 of in[0..n)                           */
+  ACC_REQ_SCAN = 1,      /* scanl1/scanr1, or scanl/scanr with a seed        */
+  ACC_REQ_FOLD_SEG = 2,  /* foldSeg/fold1Seg: aux = offsets (m + 1 values)   */
+  ACC_REQ_PERMUTE = 3,   /* permute into m slots initialised to `seed`;
+                            aux = indices (n), in = source or NULL (`fill`)  */
+  ACC_REQ_STENCIL2D = 4  /* n x m image, weighted-sum stencil              */
+} acc_request_kind;
+
+typedef struct acc_request {
+  int32_t skeleton;  /* acc_request_kind */
+  int32_t type;      /* acc_type */
+  int32_t op;        /* acc_op */
+  int32_t has_seed;
+  uint64_t seed;     /* bit pattern; permute: the defaults value */
+  uint64_t fill;     /* permute: constant source; stencil: constant boundary value */
+  const void* in;
+  int64_t n;
+  int64_t m;
+  const void* aux;
+  int32_t aux_type;  /* ACC_INT32 or ACC_INT64 */
+  int32_t direction; /* scan */
+  int32_t ry, rx;    /* stencil radii */
+  int32_t boundary;  /* stencil */
+  int32_t reserved;
+  const void* weights; /* stencil weights (host; copied at submit) */
+} acc_request;
+
+typedef struct acc_server_stats {
+  uint64_t completed, failed;
+  double p50_us, p99_us, mean_us, max_us;
+  double req_per_s, bytes_per_s, span_s; /* span: first arrival to last completion */
+  size_t arena_bytes, arena_used;
+} acc_server_stats;
+
+typedef struct acc_server_s* acc_server_t;
+
+/* hbm_budget: arena bytes (0 = half of the free device memory) */
+int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t* s);
+/* arrival = now + arrival_offset_ns; keep_output: hold the result until
+ * acc_hip_server_release (otherwise its memory is reused on completion) */
+int acc_hip_server_submit(acc_server_t s, const acc_request* r, int64_t arrival_offset_ns, int keep_output,
+                          uint64_t* id);
+int acc_hip_server_wait(acc_server_t s, uint64_t id, void** out, double* latency_us);
+int acc_hip_server_release(acc_server_t s, uint64_t id);
+int acc_hip_server_drain(acc_server_t s);
+int acc_hip_server_stats(acc_server_t s, acc_server_stats* st);
+int acc_hip_server_reset_stats(acc_server_t s);
+int acc_hip_server_destroy(acc_server_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,11 +108,12 @@ __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, in
 
 template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
 __device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t, uint32_t epoch,
-                                         int64_t* s_range, int64_t* s_starts, Maybe<SegV<T>>* s_wave,
-                                         Maybe<SegV<T>>* s_prefix) {
+                                         int64_t* s_range, int64_t* s_starts, uint32_t* s_heads,
+                                         Maybe<SegV<T>>* s_wave, Maybe<SegV<T>>* s_prefix) {
   constexpr int VEC = Vec<T>::N;
   constexpr int W = BLOCK / 64;
-  constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
+  constexpr int TILE = BLOCK * K * VEC;
+  constexpr int NBW = TILE / 32 + 1;  // head bitmap: tile positions 0..TILE
   using S = SegV<T>;
   constexpr int NW = Tagged<S>::NW;
   SegOp<Op> sop{Op{}};
@@ -120,59 +121,66 @@ __device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t
   const int lane = lane_id();
   const int w = threadIdx.x / 64;
   const int64_t base = (int64_t)t * TILE;
-  const int64_t last_e = min(base + TILE, a.e_end) - 1;
 
-  // ---- segment range of the tile, staged in LDS when it fits.  Segment g in
-  // [0, S) covers [start(g), start(g+1)); "segment" S collects elements after
-  // the last one (start(S+1) = +inf); elements before start(0) belong to
-  // "segment" -1.  Neither produces a result.
+  // ---- segment heads of the tile.  Segment g in [0, S) covers
+  // [start(g), start(g+1)); "segment" S collects the elements after the last
+  // one and elements before start(0) belong to "segment" -1 (no results).
+  // Every start in [base, base + TILE] sets its bit in an LDS bitmap; the
+  // starts themselves are staged (when they fit) for the rare end lookups.
   auto sv = [&](int64_t g) -> int64_t { return g > a.nseg ? INT64_MAX : a.st.start(g); };
-  if (threadIdx.x == 0) a.st.tile_range(t, base, last_e, a.nseg, &s_range[0], &s_range[1]);
+  if (threadIdx.x == 0) a.st.tile_range(t, base, min(base + TILE, a.e_end) - 1, a.nseg, &s_range[0], &s_range[1]);
+  for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
   __syncthreads();
   const int64_t g_lo = s_range[0], g_hi = s_range[1];
   const int64_t cnt = g_hi - g_lo + 2;  // starts g_lo .. g_hi+1
   const bool staged = cnt <= CAP;
-  if (staged)
-    for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) s_starts[i] = sv(g_lo + i);
+  for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
+    const int64_t st = sv(g_lo + i);
+    if (staged) s_starts[i] = st;
+    const int64_t p = st - base;
+    if (p >= 0 && p <= TILE) atomicOr(&s_heads[p >> 5], 1u << (p & 31));
+  }
   __syncthreads();
-  auto start_of = [&](int64_t g) -> int64_t { return staged ? s_starts[g - g_lo] : sv(g); };
-  auto search = [&](int64_t e) -> int64_t {
+  // smallest segment whose start is at least `e`
+  auto first_seg_at = [&](int64_t e) -> int64_t {
     int64_t x = g_lo, y = g_hi + 1;
     while (x < y) {
-      int64_t m = (x + y) >> 1;
-      if (start_of(m) <= e) x = m + 1; else y = m;
+      const int64_t m = (x + y) >> 1;
+      if ((staged ? s_starts[m - g_lo] : sv(m)) < e) x = m + 1; else y = m;
     }
-    return x - 1;  // >= g_lo - 1
+    return x;
   };
 
-  // ---- load and flag segment heads
-  S x[K][VEC];
-  int64_t g0[K];
+  // ---- load; segmented scan inside each thread's dwordx4 vector
+  T xv[K][VEC];
+  uint32_t hb[K], eb[K];  // per vector: head bits of its elements, "next is a head" bits
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
+    const int p0 = ((w * K + k) * 64 + lane) * VEC;
+    const int64_t e0 = base + p0;
     T v[VEC];
+    uint32_t invalid = 0;
     if (e0 + VEC <= a.e_end) {
       Vec<T> r = load16_stream(a.in + e0);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) v[j] = r.e[j];
     } else {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) v[j] = e0 + j < a.e_end ? a.in[e0 + j] : T();
+      for (int j = 0; j < VEC; ++j) {
+        const bool ok = e0 + j < a.e_end;
+        v[j] = ok ? a.in[e0 + j] : T();
+        invalid |= ok ? 0u : 1u << j;
+      }
     }
-    int64_t g = search(e0);
-    g0[k] = g;
-    int64_t cur = g >= g_lo ? start_of(g) : INT64_MIN;
-    int64_t nxt = start_of(g + 1);
+    const int sh = p0 & 31;
+    uint64_t word = s_heads[p0 >> 5];
+    if (sh + VEC + 1 > 32) word |= (uint64_t)s_heads[(p0 >> 5) + 1] << 32;
+    const uint32_t b = (uint32_t)(word >> sh) & ((1u << (VEC + 1)) - 1u);
+    hb[k] = (b & ((1u << VEC) - 1u)) | invalid;  // elements past the row start a dead segment
+    eb[k] = (b >> 1) & ~invalid;
+    xv[k][0] = v[0];
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) {
-      const int64_t e = e0 + j;
-      while (nxt <= e && g < g_hi) { ++g; cur = nxt; nxt = start_of(g + 1); }
-      x[k][j].v = v[j];
-      x[k][j].f = (e >= a.e_end || cur == e) ? 1u : 0u;
-    }
-#pragma unroll
-    for (int j = 1; j < VEC; ++j) x[k][j] = sop(x[k][j - 1], x[k][j]);
+    for (int j = 1; j < VEC; ++j) xv[k][j] = (hb[k] >> j & 1u) ? v[j] : op(xv[k][j - 1], v[j]);
   }
 
   // ---- segmented scan across lanes, chunks and waves
@@ -181,7 +189,10 @@ __device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t
   Maybe<S> run = none();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    S inc = wave_inclusive_scan(x[k][VEC - 1], sop);
+    S tot_l;
+    tot_l.v = xv[k][VEC - 1];
+    tot_l.f = hb[k] ? 1u : 0u;
+    S inc = wave_inclusive_scan(tot_l, sop);
     S up = shift_up1(inc);
     Maybe<S> exl = none();
     if (lane > 0) { exl.v = up; exl.ok = 1; }
@@ -211,9 +222,7 @@ __device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t
       Tagged<S>::publish(slot, agg.v, make_tag(epoch, ST_AGGREGATE));
     }
     // the first element needs a carry unless it starts a segment
-    const bool first_is_head = x[0][0].f != 0;  // lane 0, wave 0 holds element 0
-    const bool need = t > 0 && !__builtin_amdgcn_readfirstlane((int)first_is_head);
-    if (need) {
+    if (t > 0 && !(s_heads[0] & 1u)) {
       X.v = wave_lookback<S>(a.status, t, epoch, sop);
       X.ok = 1;
       if (!agg.v.f && lane == 0 && !last)
@@ -224,22 +233,23 @@ __device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t
   __syncthreads();
   const Maybe<S> XW = mcombine(sop, *s_prefix, wp);
 
-  // ---- results at segment ends
+  // ---- results at segment ends (the element before a head)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const Maybe<S> P = mcombine(sop, XW, pre[k]);
-    const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
-    int64_t g = g0[k];
-    int64_t nxt = start_of(g + 1);
+    if (eb[k]) {
+      // segmented prefix of everything before this vector
+      const Maybe<S> P = mcombine(sop, XW, pre[k]);
+      const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) {
-      const int64_t e = e0 + j;
-      if (e < a.e_end) {
-        while (nxt <= e && g < g_hi) { ++g; nxt = start_of(g + 1); }
-        if (nxt == e + 1 && g >= 0 && g < a.nseg) {
-          T r = P.ok ? sop(P.v, x[k][j]).v : x[k][j].v;
-          if (a.has_seed) r = op(a.seed, r);  // seed on the left
-          a.out[g] = r;
+      for (int j = 0; j < VEC; ++j) {
+        if (eb[k] >> j & 1u) {
+          const bool headed = (hb[k] & ((2u << j) - 1u)) != 0;  // a head at or before j in this vector
+          T r = (headed || !P.ok) ? xv[k][j] : op(P.v.v, xv[k][j]);
+          const int64_t g = first_seg_at(e0 + j + 1) - 1;
+          if (g >= 0 && g < a.nseg) {
+            if (a.has_seed) r = op(a.seed, r);  // seed on the left
+            a.out[g] = r;
+          }
         }
       }
     }
@@ -257,9 +267,11 @@ __device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t
 // Persistent workgroups taking tiles by ticket (next ticket prefetched).
 template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
 __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
+  constexpr int TILE = BLOCK * K * Vec<T>::N;
   __shared__ uint64_t s_tick[2];
   __shared__ int64_t s_range[2];
   __shared__ int64_t s_starts[CAP];
+  __shared__ uint32_t s_heads[TILE / 32 + 1];
   __shared__ Maybe<SegV<T>> s_wave[BLOCK / 64];
   __shared__ Maybe<SegV<T>> s_prefix;
   if (threadIdx.x == 0) s_tick[0] = take_ticket(a.ctl);
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   int cur = 0;
   while (t < a.ntiles) {
     if (threadIdx.x == 0) s_tick[cur ^ 1] = take_ticket(a.ctl);
-    seg_tile<T, Op, Starts, BLOCK, K, CAP>(a, t, epoch, s_range, s_starts, s_wave, &s_prefix);
+    seg_tile<T, Op, Starts, BLOCK, K, CAP>(a, t, epoch, s_range, s_starts, s_heads, s_wave, &s_prefix);
     __syncthreads();
     cur ^= 1;
     t = (uint32_t)s_tick[cur];
@@ -278,9 +290,12 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   block_retire(a.ctl, gridDim.x, epoch);
 }
 
+#ifndef ACC_SEG_K
+#define ACC_SEG_K 8
+#endif
 template <typename T> struct SegTune {
   static constexpr int BLOCK = 256;
-  static constexpr int K = 8;  // 32 KiB tiles
+  static constexpr int K = ACC_SEG_K;  // dwordx4 vectors per thread
   static constexpr int CAP = 1024;
 };
 

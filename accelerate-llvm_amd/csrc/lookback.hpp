@@ -111,22 +111,23 @@ __device__ __forceinline__ void block_retire(SyncCtl* ctl, uint32_t nblocks, uin
 // Several independent problems can share one status array: tile `first`
 // (the problem's first tile) always publishes ST_PREFIX and everything below
 // it is cut off.
-// `status` holds NW words per tile.  Predecessors publish ST_AGGREGATE (their
-// own reduction) early and ST_PREFIX (inclusive of everything before them)
-// once known; tile 0 publishes ST_PREFIX directly.
 //
-// The first window covers the 64 nearest predecessors (one per lane); later
-// windows give each lane S consecutive tiles (64*S per round trip), which
-// bounds the latency of a deep look-back without reading far on the common,
-// shallow one.  Lanes whose tile is not yet published re-poll with an
+// A window is C chunks of 64 consecutive predecessors, lane l of chunk k
+// holding tile lo + 64k + l, so every load instruction reads 64 adjacent
+// status words.  The look-back needs only the tiles from the nearest
+// ST_PREFIX up to tile-1: it finishes as soon as that suffix is published,
+// without waiting for older tiles.  The first window has C0 chunks, later
+// ones C1: the ST_PREFIX frontier of a long scan advances by up to 64*C
+// tiles per memory round trip.  Unpublished tiles are re-polled with an
 // exponential back-off.
-template <typename T, typename Op, int S = 4>
+template <typename T, typename Op, int C0 = 1, int C1 = 1>
 __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch, Op op, uint32_t first = 0) {
   constexpr int NW = Tagged<T>::NW;
+  constexpr int CM = C0 > C1 ? C0 : C1;
   const int lane = lane_id();
   const uint32_t tag_a = make_tag(epoch, ST_AGGREGATE);
   const uint32_t tag_p = make_tag(epoch, ST_PREFIX);
-  // segmented in-order combine: an element flagged PREFIX starts a new run
+  // segmented in-order combine: an element flagged ok starts a new run
   auto seg = [op](Maybe<T> a, Maybe<T> b) {
     Maybe<T> r;
     r.v = b.ok ? b.v : op(a.v, b.v);
@@ -136,60 +137,77 @@ __device__ T wave_lookback(const uint64_t* status, uint32_t tile, uint32_t epoch
   bool have_acc = false;
   T acc{};
   int64_t hi = (int64_t)tile - 1;  // highest predecessor not yet combined
-  int per = 1;                      // tiles per lane in this window
+  int nc = C0;
   while (true) {
-    const int64_t lo = hi - 64 * per + 1;
-    const int64_t idx0 = lo + (int64_t)lane * per;
-    T v[S];
-    bool is_p[S];
-    uint32_t ready = 0;
+    const int64_t lo = hi - 64 * nc + 1;
+    T v[CM];
+    uint32_t ready = 0, isp = 0;  // bit k: this lane's tile of chunk k is published / a PREFIX
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
+    for (int k = 0; k < CM; ++k) {
       v[k] = T{};
-      is_p[k] = false;
-      if (k >= per || idx0 + k < (int64_t)first) {
+      if (k < nc && lo + 64 * k + lane < (int64_t)first) {  // before the first tile: cut off
         ready |= 1u << k;
-        is_p[k] = idx0 + k < (int64_t)first;  // before the first tile: cut off
+        isp |= 1u << k;
       }
     }
-    int backoff = 1;
+    int backoff = 1, kf = -1, L = -1;
     while (true) {
 #pragma unroll
-      for (int k = 0; k < S; ++k) {
-        if (!(ready >> k & 1u)) {
-          const uint32_t tag = Tagged<T>::peek(status + (size_t)(idx0 + k) * NW, v[k]);
+      for (int k = 0; k < CM; ++k) {
+        if (k < nc && !(ready >> k & 1u)) {
+          const uint32_t tag = Tagged<T>::peek(status + (size_t)(lo + 64 * k + lane) * NW, v[k]);
           if (tag == tag_a || tag == tag_p) {
             ready |= 1u << k;
-            is_p[k] = tag == tag_p;
+            isp |= (tag == tag_p ? 1u : 0u) << k;
           }
         }
       }
-      const bool done = ready == (1u << S) - 1u;
-      if (__all(done)) break;
+      // walk chunks from the nearest: find the highest PREFIX above the
+      // highest unpublished tile
+      bool wait = false, found = false;
+#pragma unroll
+      for (int k = CM - 1; k >= 0; --k) {
+        if (k < nc && !found && !wait) {
+          const uint64_t R = __ballot(ready >> k & 1u), P = __ballot(isp >> k & 1u);
+          const uint64_t nr = ~R;
+          const uint64_t above = nr ? ~((2ull << (63 - __builtin_clzll(nr))) - 1ull) : ~0ull;
+          const uint64_t Pa = P & (nr && (63 - __builtin_clzll(nr)) == 63 ? 0ull : above);
+          if (Pa) {
+            found = true;
+            kf = k;
+            L = 63 - __builtin_clzll(Pa);
+          } else if (nr) {
+            wait = true;
+          }
+        }
+      }
+      if (found || !wait) break;
       if (backoff == 1) __builtin_amdgcn_s_sleep(1);
       else if (backoff == 2) __builtin_amdgcn_s_sleep(2);
       else if (backoff == 4) __builtin_amdgcn_s_sleep(4);
       else __builtin_amdgcn_s_sleep(8);
       backoff = backoff < 8 ? backoff * 2 : 8;
     }
-    // lane-level segmented fold of its tiles, in order
-    Maybe<T> m;
-    m.v = v[0];
-    m.ok = is_p[0] ? 1u : 0u;
+    // combine chunks kf..nc-1 (all chunks when no PREFIX was found) in order;
+    // in chunk kf, lane L's PREFIX cuts off the lanes below it
+    Maybe<T> w;
+    w.v = T{};
+    w.ok = 0;
+    bool wset = false;
 #pragma unroll
-    for (int k = 1; k < S; ++k) {
-      if (k < per) {
-        Maybe<T> x;
-        x.v = v[k];
-        x.ok = is_p[k] ? 1u : 0u;
-        m = seg(m, x);
+    for (int k = 0; k < CM; ++k) {
+      if (k < nc && k >= (kf < 0 ? 0 : kf)) {
+        Maybe<T> m;
+        m.v = v[k];
+        m.ok = (k == kf && lane == L) ? 1u : 0u;
+        const Maybe<T> c = read_lane(wave_inclusive_scan(m, seg), 63);
+        if (wset) w.v = op(w.v, c.v); else { w = c; wset = true; }
       }
     }
-    const Maybe<T> s = read_lane(wave_inclusive_scan(m, seg), 63);
-    if (have_acc) acc = op(s.v, acc); else { acc = s.v; have_acc = true; }
-    if (s.ok) break;
+    if (have_acc) acc = op(w.v, acc); else { acc = w.v; have_acc = true; }
+    if (kf >= 0) break;
     hi = lo - 1;
-    per = S;
+    nc = C1;
   }
   return acc;
 }

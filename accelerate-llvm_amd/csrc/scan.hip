@@ -60,9 +60,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs<T> a) {
   block_retire(a.ctl, gridDim.x, epoch);
 }
 
+#ifndef ACC_SCAN_K
+#define ACC_SCAN_K 8
+#endif
 template <typename T> struct ScanTune {
   static constexpr int BLOCK = 256;
-  static constexpr int K = 8;  // dwordx4 vectors per thread: 32 KiB tiles
+  static constexpr int K = ACC_SCAN_K;  // dwordx4 vectors per thread
 };
 
 template <typename T, typename Op, bool REVERSE, bool ALIGNED>

@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "accelerate-llvm_amd"))
 
 import acc_hip as A  # noqa: E402
+from acc_hip import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "achieved HBM GB/s (% of peak) per skeleton; served req/s + p99 latency"
@@ -58,6 +59,262 @@ def seg_bytes(n, m, b=4, ib=8):
     return n * b + (m + 1) * ib + m * b
 
 
+def bench_batched(s, args):
+    """10,000 problems: even -> Float32 scanl1 (+) on U(-1,1), odd -> Int32 fold (+) 0
+    on U[-100,100]; sizes floor(exp(U(ln 2^8, ln 2^20))).  One launch per problem
+    (the plain path) against one persistent batched launch replayed from a
+    captured hipGraph."""
+    import math
+    rng = np.random.default_rng(SEED)
+    P = args.batch_problems
+    sizes = np.floor(np.exp(rng.uniform(math.log(2 ** 8), math.log(2 ** 20), size=P))).astype(np.int64)
+    pad = lambda k: (k + 63) // 64 * 64  # 256-byte aligned sub-arrays
+    scan_idx = np.arange(0, P, 2)
+    fold_idx = np.arange(1, P, 2)
+    s_off = np.concatenate([[0], np.cumsum([pad(int(sizes[i])) for i in scan_idx])])
+    f_off = np.concatenate([[0], np.cumsum([pad(int(sizes[i])) for i in fold_idx])])
+    log(f"batched: {P} problems, {int(sizes.sum())} elements; generating inputs")
+    xs = W.uniform_f32(SEED, -1.0, 1.0, int(s_off[-1]))
+    xf = W.uniform_i32(SEED + 1, -100, 100, int(f_off[-1]))
+    d_xs, d_xf = A.DeviceArray.from_numpy(s, xs), A.DeviceArray.from_numpy(s, xf)
+    d_ys = A.DeviceArray(int(s_off[-1]), np.float32)
+    d_yf = A.DeviceArray(len(fold_idx) * 64, np.int32)
+    probs = []
+    for j, i in enumerate(scan_idx):
+        n = int(sizes[i])
+        probs.append((A.BATCH_SCAN, A.ADD, None, d_xs.view(int(s_off[j]), n), n, d_ys.view(int(s_off[j]), n)))
+    for j, i in enumerate(fold_idx):
+        n = int(sizes[i])
+        probs.append((A.BATCH_FOLD, A.ADD, np.int32(0), d_xf.view(int(f_off[j]), n), n, d_yf.view(j * 64, 1)))
+
+    def per_call():
+        for (kind, op, seed, src, n, out) in probs:
+            if kind == A.BATCH_SCAN:
+                A.scan(s, op, A.LEFT_TO_RIGHT, None, src, 1, n, out)
+            else:
+                A.fold(s, op, seed, src, 1, n, out)
+
+    per_call()
+    s.synchronize()
+    e0, e1 = A.Event(0), A.Event(0)
+    w0 = time.perf_counter()
+    e0.record(s)
+    per_call()
+    e1.record(s)
+    e1.synchronize()
+    percall_wall = time.perf_counter() - w0
+    percall_ms = e0.elapsed_ms(e1)
+    ref_fold = d_yf.to_numpy(s)[::64].copy()
+    t0 = time.perf_counter()
+    b = A.Batch(0, probs, tile=0)
+    tune_s = time.perf_counter() - t0
+    k, tiles = b.tile()
+    g = A.Graph.capture(s, lambda: b.launch(s))
+    g.launch(s)
+    s.synchronize()
+    # every problem equals its standalone result (same algorithm per element for
+    # the integer folds; float scans within the FP32 bound of each other)
+    got_fold = d_yf.to_numpy(s)[::64]
+    assert np.array_equal(got_fold, ref_fold), "batched fold differs from the plain calls"
+    got_scan = d_ys.to_numpy(s)
+    sample = scan_idx[:: max(1, len(scan_idx) // 50)]
+    for i in sample:
+        j = int(np.searchsorted(scan_idx, i))
+        n = int(sizes[i])
+        x64 = xs[int(s_off[j]):int(s_off[j]) + n].astype(np.float64)
+        a = got_scan[int(s_off[j]):int(s_off[j]) + n].astype(np.float64)
+        kk = np.arange(1, n + 1)
+        bound = 1e-5 * np.maximum(1, np.log2(np.maximum(kk, 2))) * np.cumsum(np.abs(x64)) + 1e-12
+        assert np.all(np.abs(a - np.cumsum(x64)) <= bound), "batched scan outside the FP32 bound"
+    want = np.sum(xf[int(f_off[0]):int(f_off[0]) + int(sizes[fold_idx[0]])], dtype=np.int32)
+    assert got_fold[0] == want, "batched fold"
+    reps = 10
+    e0.record(s)
+    for _ in range(reps):
+        g.launch(s)
+    e1.record(s)
+    e1.synchronize()
+    batched_ms = e0.elapsed_ms(e1) / reps
+    nbytes = sum(int(sizes[i]) * 8 for i in scan_idx) + sum(int(sizes[i]) * 4 + 4 for i in fold_idx)
+    out = {
+        "problems": P,
+        "elements": int(sizes.sum()),
+        "bytes": nbytes,
+        "per_call_ms": round(percall_ms, 3),
+        "per_call_wall_ms": round(percall_wall * 1e3, 3),
+        "batched_ms": round(batched_ms, 3),
+        "speedup": round(percall_ms / batched_ms, 2),
+        "batched_gbps": round(nbytes / (batched_ms * 1e-3) / 1e9, 1),
+        "tile_vectors_per_thread": k,
+        "tiles": tiles,
+        "autotune_s": round(tune_s, 2),
+    }
+    log(f"batched: per-call {percall_ms:.2f} ms (wall {percall_wall*1e3:.1f} ms), batched {batched_ms:.3f} ms "
+        f"-> {out['speedup']}x, {out['batched_gbps']} GB/s (tile K={k})")
+    g.destroy()
+    b.destroy()
+    return out
+
+
+def bench_serving(s, args):
+    """Poisson arrivals of mixed requests (35% scanl1 (+), 30% fold1Seg (+), 20%
+    histogram permute into 4096 bins, 15% 3x3 mirror stencil), sizes
+    log-uniform in [2^10, 2^24], element type uniform over Float32/Int32/Int64,
+    inputs resident on the device, admission by free HBM."""
+    import math
+    R = args.serve_requests
+    rng = np.random.default_rng(SEED + 7)
+    N = 1 << 24
+    pool = {
+        "f32": A.DeviceArray.from_numpy(s, W.uniform_f32(SEED, -1.0, 1.0, N)),
+        "i32": A.DeviceArray.from_numpy(s, W.uniform_i32(SEED, -100, 100, N)),
+        "i64": A.DeviceArray.from_numpy(s, W.uniform_i32(SEED, -100, 100, N).astype(np.int64)),
+    }
+    lens = W.segment_lengths(SEED, N // 1024, 16384, N)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    d_off = A.DeviceArray.from_numpy(s, off)
+    d_idx = A.DeviceArray.from_numpy(s, W.zipf_i32(SEED, 1.1, 4096, N))
+    dts = {"f32": np.float32, "i32": np.int32, "i64": np.int64}
+    reqs, nbytes = [], 0.0
+    for i in range(R):
+        key = ["f32", "i32", "i64"][rng.integers(0, 3)]
+        dt, src = dts[key], pool[key]
+        kind = rng.choice(4, p=[0.35, 0.30, 0.20, 0.15])
+        n = int(math.exp(rng.uniform(math.log(2 ** 10), math.log(2 ** 24))))
+        B = np.dtype(dt).itemsize
+        if kind == 0:
+            reqs.append(A.Server.request(A.REQ_SCAN, dt, A.ADD, src=src, n=n))
+            nbytes += 2 * n * B
+        elif kind == 1:
+            nseg = max(1, int(np.searchsorted(off, n, side="right")) - 1)
+            ne = int(off[nseg])
+            reqs.append(A.Server.request(A.REQ_FOLD_SEG, dt, A.ADD, src=src, n=ne, m=nseg, aux=d_off))
+            nbytes += ne * B + (nseg + 1) * 8 + nseg * B
+        elif kind == 2:
+            reqs.append(A.Server.request(A.REQ_PERMUTE, dt, A.ADD, seed=0, n=n, m=4096, aux=d_idx, fill=1))
+            nbytes += n * 4 + 2 * 4096 * B
+        else:
+            side = max(3, int(math.sqrt(n)))
+            w = (np.outer([1, 2, 1], [1, 2, 1]) / (16 if key == "f32" else 1)).astype(dt)
+            reqs.append(A.Server.request(A.REQ_STENCIL2D, dt, src=src, n=side, m=side, ry=1, rx=1,
+                                         boundary=A.MIRROR, weights=w))
+            nbytes += 2 * side * side * B
+
+    # the plain per-call path: each request runs alone, the client waits for it
+    plain_t0 = time.perf_counter()
+    outs = A.DeviceArray(N * 8, np.uint8)
+    for (r, w) in reqs:
+        run_plain(s, r, w, outs)
+        s.synchronize()
+    plain_s = time.perf_counter() - plain_t0
+
+    srv = A.Server(0, slots=8)
+    # capacity: every request arrives at once
+    for q in reqs:
+        srv.submit(q)
+    srv.drain()
+    cap = srv.stats()
+    capacity = cap.req_per_s
+    srv.reset_stats()
+    lam = 0.8 * capacity
+    t = 0.0
+    for q in reqs:
+        t += rng.exponential(1.0 / lam)
+        srv.submit(q, arrival_offset_ns=int(t * 1e9))
+    srv.drain()
+    st = srv.stats()
+    srv.destroy()
+    out = {
+        "requests": R,
+        "slots": 8,
+        "capacity_req_per_s": round(capacity, 1),
+        "arrival_rate_req_per_s": round(lam, 1),
+        "req_per_s": round(st.req_per_s, 1),
+        "p50_ms": round(st.p50_us / 1e3, 3),
+        "p99_ms": round(st.p99_us / 1e3, 3),
+        "mean_ms": round(st.mean_us / 1e3, 3),
+        "gbps": round(st.bytes_per_s / 1e9, 1),
+        "plain_req_per_s": round(R / plain_s, 1),
+        "capacity_vs_plain": round(capacity / (R / plain_s), 2),
+    }
+    log(f"serving: capacity {capacity:.0f} req/s (plain per-call {R/plain_s:.0f} req/s); at {lam:.0f} req/s "
+        f"p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, {out['req_per_s']} req/s")
+    return out
+
+
+def run_plain(s, r, w, out):
+    """One request through the plain C-ABI call (what the server runs per request)."""
+    L = A.lib()
+    seed = A.ctypes.c_uint64(r.seed)
+    fill = A.ctypes.c_uint64(r.fill)
+    sp = A.ctypes.byref(seed) if r.has_seed else None
+    if r.skeleton == A.REQ_SCAN:
+        A.check(L.acc_hip_scan(s.handle, r.type, r.op, r.direction, sp, r.in_, 1, r.n, A.c_void_p(out.ptr)))
+    elif r.skeleton == A.REQ_FOLD_SEG:
+        A.check(L.acc_hip_fold_seg(s.handle, r.type, r.op, sp, r.in_, 1, r.n, r.aux_type, r.aux, r.m,
+                                   A.c_void_p(out.ptr)))
+    elif r.skeleton == A.REQ_PERMUTE:
+        sz = 8 if r.type in (A.INT64, A.FLOAT64, A.WORD64) else 4
+        A.check(L.acc_hip_fill_async(s.handle, A.c_void_p(out.ptr), A.ctypes.byref(seed), sz, r.m))
+        A.check(L.acc_hip_permute(s.handle, r.type, r.op, A.c_void_p(out.ptr), A.c_void_p(out.ptr), r.m, r.in_,
+                                  A.ctypes.byref(fill), r.aux_type, r.aux, r.n))
+    else:
+        A.check(L.acc_hip_stencil2d(s.handle, r.type, r.ry, r.rx, w.ctypes.data_as(A.c_void_p), r.boundary,
+                                    A.ctypes.byref(fill), r.in_, r.n, r.m, A.c_void_p(out.ptr)))
+
+
+def kernel_only():
+    """The dominant kernel (fold1Seg (+) on the step's inputs) five times and a
+    1 GiB device copy three times: the program the counter runs profile."""
+    s = A.Stream(0)
+    lens = W.segment_lengths(SEED, N_SEG, MAX_LEN, N_VALUES)
+    off = W.offsets(lens)
+    d_vals = A.DeviceArray.from_numpy(s, W.normal_f32(SEED, N_VALUES))
+    d_off = A.DeviceArray.from_numpy(s, off)
+    d_sum = A.DeviceArray(N_SEG, np.float32)
+    for _ in range(5):
+        A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)
+    src, dst = A.DeviceArray(1 << 28, np.int32), A.DeviceArray(1 << 28, np.int32)
+    for _ in range(3):
+        A.check(A.lib().acc_hip_memcpy_d2d_async(s.handle, A.c_void_p(dst.ptr), A.c_void_p(src.ptr), 1 << 30))
+    s.synchronize()
+    return 0
+
+
+def measure_traffic():
+    """HBM bytes per fold1Seg (+) launch from two rocprofv3 counter runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled so that the 1 GiB
+    device copy in the same run reads/writes exactly 1 GiB.  None on failure."""
+    import csv
+    import subprocess
+    import tempfile
+    vals = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="acc_pmc_")
+        cmd = ["rocprofv3", "--pmc", counter, "-T", "-f", "csv", "-d", d, "-o", "k", "--",
+               sys.executable, os.path.abspath(__file__), "--kernel-only"]
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120, check=False)
+        except (subprocess.TimeoutExpired, OSError):
+            return None
+        files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        if not files:
+            return None
+        seg, cp = [], []
+        for row in csv.DictReader(open(files[0])):
+            if row["Counter_Name"] != counter:
+                continue
+            if row["Kernel_Name"].startswith("k_fold_seg"):
+                seg.append(float(row["Counter_Value"]))
+            elif "copyBuffer" in row["Kernel_Name"] and int(row["Grid_Size"]) >= 65536:
+                cp.append(float(row["Counter_Value"]))
+        if not seg or not cp:
+            return None
+        scale = (1 << 30) / (sum(cp) / len(cp) * 1024.0)
+        vals[counter] = sum(seg) / len(seg) * 1024.0 * scale
+    return int(vals["FETCH_SIZE"] + vals["WRITE_SIZE"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +324,13 @@ def main():
     ap.add_argument("--no-skeletons", action="store_true", help="skip the per-skeleton lines")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=10000)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="(internal) run only the dominant kernel and a 1 GiB copy, for counter collection")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -77,8 +341,9 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as O  # input generators; the oracle is also the CPU baseline below
+
+    if args.kernel_only:
+        return kernel_only()
 
     s = A.Stream(0)
     props = A.device_props(0)
@@ -86,8 +351,8 @@ def main():
 
     # ---------------------------------------------------------------- inputs
     t0 = time.time()
-    lens = O.gen_segment_lengths(SEED, N_SEG, MAX_LEN, N_VALUES)
-    vals = O.gen_normal_f32(SEED, N_VALUES)
+    lens = W.segment_lengths(SEED, N_SEG, MAX_LEN, N_VALUES)
+    vals = W.normal_f32(SEED, N_VALUES)
     d_lens = A.DeviceArray.from_numpy(s, lens)
     d_vals = A.DeviceArray.from_numpy(s, vals)
     d_off = A.DeviceArray(N_SEG + 1, np.int64)
@@ -106,15 +371,16 @@ def main():
         step()
     s.synchronize()
 
-    # parity of the step's outputs (cheap: the checker runs once)
-    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    # sanity of the step's outputs with numpy (the parity tests cover the rest)
+    off = W.offsets(lens)
     assert np.array_equal(d_off.to_numpy(s), off), "offsets"
-    assert np.array_equal(d_max.to_numpy(s), O.fold_seg(A.MAX, None, vals, off)), "fold1Seg max"
-    truth, ab = O.fold_seg_f64(vals, off)
+    assert np.array_equal(d_max.to_numpy(s), np.maximum.reduceat(vals, off[:-1])), "fold1Seg max"
+    truth = np.add.reduceat(vals.astype(np.float64), off[:-1])
+    ab = np.add.reduceat(np.abs(vals.astype(np.float64)), off[:-1])
     bound = 1e-5 * np.maximum(1.0, np.log2(np.maximum(np.diff(off), 2))) * ab
     err = np.abs(d_sum.to_numpy(s).astype(np.float64) - truth)
     assert np.all(err <= bound), "fold1Seg (+) outside the FP32 bound"
-    log(f"step outputs match the oracle (max rel-to-bound {float((err / bound).max()):.3f})")
+    log(f"step outputs checked (fold1Seg (+) max error/bound {float((err / bound).max()):.3f})")
 
     # ------------------------------------------------------------ timed steps
     e0, e1 = A.Event(0), A.Event(0)
@@ -146,6 +412,11 @@ def main():
         "avg_us": round(k_ms * 1e3, 2),
     }
     log(f"fold1Seg (+) kernel {k_ms * 1e3:.1f} us -> {k_gbs:.0f} GB/s")
+    if not args.no_traffic:
+        roofline["traffic"] = measure_traffic()
+        if roofline["traffic"]:
+            log(f"fold1Seg (+) HBM traffic {roofline['traffic']} B per launch "
+                f"({roofline['traffic'] / k_bytes:.3f}x algorithmic)")
 
     # --------------------------------------------------- per-skeleton lines
     skel = {}
@@ -159,7 +430,7 @@ def main():
             log(f"{name}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
 
         n28 = 1 << 28
-        x = O.gen_uniform_i32(SEED, -1000, 1000, n28)
+        x = W.uniform_i32(SEED, -1000, 1000, n28)
         dx = A.DeviceArray.from_numpy(s, x)
         del x
         dy = A.DeviceArray(n28, np.int32)
@@ -176,7 +447,7 @@ def main():
         record("fold1Seg_max_f32_2^26", k_bytes,
                timed(s, lambda: A.fold_seg(s, A.MAX, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_max), R))
 
-        idx = O.gen_zipf_i32(SEED, 1.1, 4096, 1 << 26)
+        idx = W.zipf_i32(SEED, 1.1, 4096, 1 << 26)
         d_idx = A.DeviceArray.from_numpy(s, idx)
         del idx
         bins = A.DeviceArray(4096, np.int32)
@@ -196,9 +467,18 @@ def main():
                timed(s, lambda: A.stencil2d(s, 1, 1, w3, A.MIRROR, None, d_img, H, W, d_out), R))
         del d_out
 
+    # ------------------------------------------------------- batched (C4)
+    batched = None if args.no_batched else bench_batched(s, args)
+    # ------------------------------------------------------- serving (C5)
+    serving = None if args.no_serving else bench_serving(s, args)
+
     # ------------------------------------------------------------ CPU baseline
     cpu = None
     if not args.no_cpu_baseline:
+        # the CPU baseline is the oracle's restatement of the native backend's
+        # schedule, timed on this host's cores
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
         cores = len(os.sched_getaffinity(0))
         out_s = np.empty(N_SEG, np.float32)
         out_m = np.empty(N_SEG, np.float32)
@@ -243,6 +523,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "skeletons": skel,
+        "batched": batched,
+        "serving": serving,
         "device": {"name": props.name.decode(), "arch": props.arch.decode(), "cus": props.compute_units},
     }
     print(json.dumps(result), flush=True)

@@ -60,3 +60,21 @@ def test_bad_device_index_is_rejected():
     assert L.acc_hip_set_alloc_limit(-1, 0) == acc_hip.INVALID_VALUE
     p = ctypes.c_void_p()
     assert L.acc_hip_malloc(999, 16, ctypes.byref(p)) != acc_hip.SUCCESS
+
+
+def test_workload_generators():
+    from acc_hip import workloads as W
+    import numpy as np
+    u = W.uniform_i32(42, -1000, 1000, 1 << 16)
+    assert u.dtype == np.int32 and u.min() == -1000 and u.max() == 1000
+    assert np.array_equal(u, W.uniform_i32(42, -1000, 1000, 1 << 16))
+    lens = W.segment_lengths(42, 1 << 12, 16384, 1 << 22)
+    assert lens.sum() == 1 << 22 and lens.min() >= 1
+    off = W.offsets(lens)
+    assert off[0] == 0 and off[-1] == 1 << 22 and np.all(np.diff(off) >= 1)
+    z = W.zipf_i32(42, 1.1, 4096, 1 << 20)
+    p1 = 1.0 / sum(k ** -1.1 for k in range(1, 4097))
+    assert z.min() >= 0 and z.max() < 4096 and abs((z == 0).mean() - p1) < 0.01
+    f = W.uniform_f32(1, -1.0, 1.0, 1000)
+    assert f.dtype == np.float32 and f.min() >= -1 and f.max() < 1
+    assert W.normal_f32(3, 10).dtype == np.float32

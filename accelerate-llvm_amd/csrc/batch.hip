@@ -57,7 +57,7 @@ template <typename T> __device__ __forceinline__ T bits_to(uint64_t b) {
 // Fold of one tile; the problem's last tile finishes the fold.
 template <typename T, typename Op, int BLOCK, int K, bool MASKED>
 __device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, uint32_t t, uint32_t epoch,
-                                          Maybe<T>* s_wave) {
+                                          T* s_wave) {
   constexpr int VEC = Vec<T>::N;
   constexpr int W = BLOCK / 64;
   constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
@@ -66,7 +66,8 @@ __device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, 
   const int lane = lane_id(), w = threadIdx.x / 64;
   const T* in = (const T*)d.in;
   const int64_t base = (int64_t)t * TILE;
-  Maybe<T> acc = none<T>();
+  const T id = identity<T>(op);
+  T acc = id;
   if (!MASKED) {
     Vec<T> v[K];
 #pragma unroll
@@ -78,29 +79,29 @@ __device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, 
     for (int k = 1; k < K; ++k)
 #pragma unroll
       for (int j = 0; j < VEC; ++j) r = op(r, v[k].e[j]);
-    acc = Maybe<T>{r, 1u};
+    acc = r;
   } else {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
 #pragma unroll
       for (int j = 0; j < VEC; ++j)
-        if (e0 + j < d.n) acc = mcombine(op, acc, Maybe<T>{in[e0 + j], 1u});
+        acc = op(acc, e0 + j < d.n ? in[e0 + j] : id);
     }
   }
-  const Maybe<T> wr = wave_reduce_m(acc, op);
+  const T wr = wave_reduce(acc, op);
   if (lane == 0) s_wave[w] = wr;
   __syncthreads();
   if (w == 0) {
-    Maybe<T> agg = none<T>();
+    T agg = id;
 #pragma unroll
-    for (int i = 0; i < W; ++i) agg = mcombine(op, agg, s_wave[i]);
+    for (int i = 0; i < W; ++i) agg = op(agg, s_wave[i]);
     uint64_t* slot = status + (size_t)t * NW;
     const bool first = t == 0, last = t + 1 == d.ntiles;
     if (!last) {
-      if (lane == 0) Tagged<T>::publish(slot, agg.v, make_tag(epoch, first ? ST_PREFIX : ST_AGGREGATE));
+      if (lane == 0) Tagged<T>::publish(slot, agg, make_tag(epoch, first ? ST_PREFIX : ST_AGGREGATE));
     } else {
-      T r = agg.v;
+      T r = agg;
       if (!first) r = op(wave_lookback<T>(status, t, epoch, op), r);
       if (d.has_seed) r = op(bits_to<T>(d.seed), r);  // seed on the left
       if (lane == 0) *(T*)d.out = r;
@@ -112,8 +113,8 @@ template <typename T, typename Op, int BLOCK, int K>
 __device__ __forceinline__ void batch_tile(const BatchDesc& d, int kind, uint64_t* status, uint32_t t,
                                            uint32_t epoch, unsigned char* s_raw) {
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
-  Maybe<T>* s_wave = (Maybe<T>*)s_raw;
-  Maybe<T>* s_prefix = s_wave + BLOCK / 64;
+  T* s_wave = (T*)s_raw;
+  T* s_prefix = s_wave + BLOCK / 64;
   const int64_t base = (int64_t)t * TILE;
   const bool full = base + TILE <= d.n;
   if (kind == BK_SCAN) {
@@ -130,16 +131,14 @@ __device__ __forceinline__ void batch_tile(const BatchDesc& d, int kind, uint64_
 
 template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
-  __shared__ uint64_t s_tick[2];
-  __shared__ __align__(16) unsigned char s_raw[(BLOCK / 64 + 1) * sizeof(Maybe<double>)];
-  if (threadIdx.x == 0) s_tick[0] = take_ticket(a.ctl);
+  __shared__ uint64_t s_tick;
+  __shared__ __align__(16) unsigned char s_raw[(BLOCK / 64 + 1) * sizeof(double)];
+  if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
   __syncthreads();
-  const uint64_t tk0 = s_tick[0];
+  const uint64_t tk0 = s_tick;
   const uint32_t epoch = (uint32_t)(tk0 >> 32);
   uint32_t g = (uint32_t)tk0;
-  int cur = 0;
   while (g < a.total_tiles) {
-    if (threadIdx.x == 0) s_tick[cur ^ 1] = take_ticket(a.ctl);
     const BatchDesc d = a.desc[a.tile_prob[g]];
     const uint32_t t = g - d.tile0;
     uint64_t* st = a.status + d.status_off;
@@ -155,8 +154,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
       default: batch_tile<double, OpMax, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
     }
     __syncthreads();
-    cur ^= 1;
-    g = (uint32_t)s_tick[cur];
+    if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
+    __syncthreads();
+    g = (uint32_t)s_tick;
   }
   block_retire(a.ctl, gridDim.x, epoch);
 }

@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "../../include/accelerate_hip.h"
 
 namespace acc {
@@ -68,10 +69,76 @@ template <typename Op> struct Flip {
   template <typename T> __host__ __device__ __forceinline__ T operator()(T a, T b) const { return op(b, a); }
 };
 
-// A value that may be absent. Reductions without an identity element carry
-// the "have a value" bit through the wave and block stages so that the
-// ragged edges of a tile need no neutral element (fold1, scanl1 and the
-// NaN-propagation rules of fminf/fmaxf all stay exact).
+// Two-sided identity of each combine: op(e, x) == x == op(x, e) exactly, for
+// every x including NaNs and signed zeros.  Used to pad the ragged edge of a
+// tile.  Floating (+) uses -0.0 (-0.0 + x == x also for x == -0.0); fminf and
+// fmaxf return the other operand when one is a quiet NaN (C99 7.12.12), so a
+// quiet NaN is their identity.
+template <typename T> struct Limits;
+template <> struct Limits<int32_t> {
+  static __device__ __forceinline__ int32_t lo() { return INT32_MIN; }
+  static __device__ __forceinline__ int32_t hi() { return INT32_MAX; }
+};
+template <> struct Limits<int64_t> {
+  static __device__ __forceinline__ int64_t lo() { return INT64_MIN; }
+  static __device__ __forceinline__ int64_t hi() { return INT64_MAX; }
+};
+template <> struct Limits<uint32_t> {
+  static __device__ __forceinline__ uint32_t lo() { return 0u; }
+  static __device__ __forceinline__ uint32_t hi() { return UINT32_MAX; }
+};
+template <> struct Limits<uint64_t> {
+  static __device__ __forceinline__ uint64_t lo() { return 0ull; }
+  static __device__ __forceinline__ uint64_t hi() { return UINT64_MAX; }
+};
+
+template <typename T> __device__ __forceinline__ T neg_zero() { return T(0); }
+template <> __device__ __forceinline__ float neg_zero<float>() { return __int_as_float(0x80000000); }
+template <> __device__ __forceinline__ double neg_zero<double>() {
+  return __longlong_as_double((long long)0x8000000000000000ull);
+}
+template <typename T> __device__ __forceinline__ T quiet_nan() { return T(0); }
+template <> __device__ __forceinline__ float quiet_nan<float>() { return __int_as_float(0x7fc00000); }
+template <> __device__ __forceinline__ double quiet_nan<double>() {
+  return __longlong_as_double((long long)0x7ff8000000000000ull);
+}
+
+template <typename T, typename Op> struct Ident;
+template <typename T> struct Ident<T, OpAdd> {
+  static __device__ __forceinline__ T value() { return neg_zero<T>(); }
+};
+template <typename T> struct Ident<T, OpMul> {
+  static __device__ __forceinline__ T value() { return T(1); }
+};
+template <typename T> struct Ident<T, OpMin> {
+  static __device__ __forceinline__ T value() {
+    if constexpr (std::is_floating_point<T>::value) return quiet_nan<T>();
+    else return Limits<T>::hi();
+  }
+};
+template <typename T> struct Ident<T, OpMax> {
+  static __device__ __forceinline__ T value() {
+    if constexpr (std::is_floating_point<T>::value) return quiet_nan<T>();
+    else return Limits<T>::lo();
+  }
+};
+template <typename T> struct Ident<T, OpAnd> {
+  static __device__ __forceinline__ T value() { return (T)~(T)0; }
+};
+template <typename T> struct Ident<T, OpOr> {
+  static __device__ __forceinline__ T value() { return T(0); }
+};
+template <typename T> struct Ident<T, OpXor> {
+  static __device__ __forceinline__ T value() { return T(0); }
+};
+template <typename T, typename Op> struct Ident<T, Flip<Op>> {
+  static __device__ __forceinline__ T value() { return Ident<T, Op>::value(); }
+};
+template <typename T, typename Op> __device__ __forceinline__ T identity(Op) { return Ident<T, Op>::value(); }
+
+// A value that may be absent: the "have a value" bit travels through wave and
+// block stages where a lane or a tile may hold no element (fold1 of ragged
+// tiles, the segmented look-back).
 template <typename T> struct Maybe {
   T v;
   uint32_t ok;

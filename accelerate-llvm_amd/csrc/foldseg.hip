@@ -1,5 +1,5 @@
 // foldSeg / fold1Seg, and fold along the innermost dimension of a rank >= 2
-// array (every row one segment), in ONE load-balanced launch.
+// array (every row one segment), load-balanced over equal tiles.
 //
 // The reference assigns one block or one warp per segment
 // (PTX/CodeGen/FoldSeg.hs:79-264 and :274-458, selected at
@@ -8,10 +8,22 @@
 // idle most lanes.  Here the elements are cut into equal tiles regardless of
 // segment boundaries and the fold becomes a segmented scan over
 // (head-flag, value) pairs, evaluated in registers; the value at the last
-// element of each segment is its result.  A segment that crosses tiles takes
-// its carry-in from a decoupled look-back in which every tile that contains a
-// segment head publishes its final prefix immediately, so look-back depth is
-// at most the number of tiles a segment spans.
+// element of each segment is its result.
+//
+// Three launches, none of which waits on another workgroup:
+//   k_tile_seg  (offsets only) per tile: its first segment and the positions
+//               of up to SLOT segment starts inside it, so a tile stages its
+//               segment heads with one load issued ahead of its data; also
+//               writes the seed into empty segments;
+//   k_fold_seg  one workgroup per tile: every segment end whose segment
+//               starts inside the tile is final at once and written; the one
+//               end that continues a segment from earlier tiles ("open") and
+//               the tile's segmented aggregate go to a per-tile record;
+//   k_seg_fix   one workgroup: segmented scan over the tile aggregates, which
+//               completes the open ends (at most one per tile).
+// A single-pass look-back would chain every tile to its predecessor; with
+// segments shorter than a tile almost every tile needs a carry, and the
+// chain of round trips, not HBM, then bounds the launch.
 //
 // Segments are given as offsets (the `scanl (+) 0` of the lengths; the
 // frontend has already applied it, PTX/Execute.hs:359,
@@ -22,8 +34,9 @@
 //
 // HBM roofline: n*sizeof(T) + (m+1)*sizeof(I) + m*sizeof(T) bytes.
 #include "dispatch.hpp"
-#include "lookback.hpp"
+#include "host.hpp"
 #include "vecmem.hpp"
+#include "wave64.hpp"
 
 namespace acc {
 
@@ -43,57 +56,79 @@ template <typename Op> struct SegOp {
   }
 };
 
-// Segment starts given by an offsets array: start(g) = off[g], g in [0, S].
-// The segment range of each tile comes from a pre-pass (k_tile_seg).
-template <typename I> struct OffsetStarts {
-  const I* off;
-  const int64_t* tseg;  // tseg[T] = last g with start(g) <= T * TILE (>= 0)
-  __device__ __forceinline__ int64_t start(int64_t g) const { return (int64_t)off[g]; }
-  __device__ __forceinline__ void tile_range(uint32_t t, int64_t, int64_t, int64_t, int64_t* lo, int64_t* hi) const {
-    *lo = tseg[t];
-    *hi = tseg[t + 1];  // an upper bound of the segment of the tile's last element
-  }
+// Element positions inside the kernels are relative to the 16-byte boundary
+// at or below the row's first element: position p is row element p - shift,
+// and positions below `shift` are padding (dead elements).  So every tile and
+// every dwordx4 load is aligned, also for sub-array views and for the rows
+// of a rank-2 array.
+
+constexpr int SLOT = 64;  // segment starts per tile handed over by k_tile_seg
+
+// Per-tile segment range from k_tile_seg: starts g_lo .. g_lo + cnt - 1 cover
+// the tile (g_lo: the last segment starting at or before its first position,
+// 0 if none; the last one starts after its end or is past the last segment).
+struct TileSegs {
+  int64_t g_lo;
+  int64_t cnt;
 };
 
-// foldDim: start(g) = g * len.
+// Per-tile record for k_seg_fix.
+template <typename T> struct TileCarry {
+  SegV<T> agg;     // segmented combination of the whole tile
+  T open_v;        // the tile's first elements up to the end of their segment
+  int64_t open_g;  // that segment, or -1 when the tile has no open end
+};
+
+// Segment starts given by an offsets array: start(g) = off[g] + shift, g in [0, S].
+template <typename I> struct OffsetStarts {
+  const I* off;
+  const TileSegs* segs;
+  const int32_t* slots;  // SLOT tile-relative starts per tile (valid when cnt <= SLOT)
+  int64_t shift;
+  __device__ __forceinline__ int64_t start(int64_t g) const { return (int64_t)off[g] + shift; }
+};
+
+// foldDim: start(g) = g * len + shift.
 struct UniformStarts {
   int64_t len;
-  __device__ __forceinline__ int64_t start(int64_t g) const { return g * len; }
-  __device__ __forceinline__ void tile_range(uint32_t, int64_t first, int64_t last, int64_t nseg, int64_t* lo,
-                                             int64_t* hi) const {
-    *lo = first / len;
-    *hi = min(last / len, nseg);
-  }
+  int64_t shift;
+  __device__ __forceinline__ int64_t start(int64_t g) const { return g * len + shift; }
 };
 
 template <typename T, typename Starts> struct SegArgs {
-  const T* in;
+  const T* in;  // 16-byte aligned: position 0
   T* out;
   Starts st;
   int64_t nseg;   // S
-  int64_t e_end;  // elements per row
+  int64_t e_lo;   // position of the row's first element (the shift)
+  int64_t e_end;  // one past the position of the row's last element
   uint32_t ntiles;
   uint32_t has_seed;
   T seed;
-  uint32_t check_empty;  // look for empty segments (offsets only)
-  SyncCtl* ctl;
-  uint64_t* status;
+  TileCarry<T>* carry;
 };
 
-// One wave per tile boundary T in [0, ntiles]: tseg[T] = last g in [0, S]
-// with off[g] <= T * tile (0 if none), by a 64-ary search (one round trip
-// narrows the range 64-fold).
-template <typename I>
+// A tile-relative position clamped to [-1, tile + 1]: order against the
+// tile's own positions is kept.
+__device__ __forceinline__ int32_t rel_pos(int64_t p, int64_t base, int64_t tile) {
+  const int64_t r = p - base;
+  return r < 0 ? -1 : (r > tile ? (int32_t)tile + 1 : (int32_t)r);
+}
+
+// One wave per tile: the tile's segment range by two 64-ary searches (one
+// round trip narrows a range 64-fold) and, when they fit, the tile-relative
+// positions of its starts.  Then, when a seed is given, every empty segment
+// gets the seed (grid-stride over the segments).
+template <typename T, typename I>
 __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, int64_t tile, uint32_t ntiles,
-                                                  int64_t* tseg) {
-  const uint32_t T = blockIdx.x * 4u + threadIdx.x / 64u;
-  if (T > ntiles) return;  // whole waves
+                                                  int64_t shift, TileSegs* segs, int32_t* slots, T* out,
+                                                  uint32_t fill_empty, T seed) {
+  const uint32_t t = blockIdx.x * 4u + threadIdx.x / 64u;
   const int lane = lane_id();
-  const int64_t e = (int64_t)T * tile;
-  int64_t lo = 0, hi = nseg;
-  if ((int64_t)off[0] > e) {
-    hi = 0;  // no segment starts at or before e
-  } else {
+  // last g in [0, nseg] with off[g] <= e, 0 if none
+  auto search = [&](int64_t e) -> int64_t {
+    if ((int64_t)off[0] > e) return 0;
+    int64_t lo = 0, hi = nseg;
     while (hi > lo) {
       const int64_t step = (hi - lo + 64) / 64;
       const int64_t g = lo + (int64_t)lane * step;
@@ -102,201 +137,271 @@ __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, in
       lo = lo + (int64_t)(c - 1) * step;
       hi = min(hi, lo + step - 1);
     }
+    return lo;
+  };
+  if (t < ntiles) {  // whole waves
+    const int64_t base = (int64_t)t * tile;  // position
+    const int64_t g_lo = search(base - shift);
+    const int64_t g_hi = search(base + tile - shift);
+    const int64_t cnt = g_hi - g_lo + 2;
+    if (lane == 0) segs[t] = TileSegs{g_lo, cnt};
+    if (cnt <= SLOT && lane < cnt) {
+      const int64_t g = g_lo + lane;
+      slots[(size_t)t * SLOT + lane] = g > nseg ? (int32_t)tile + 1 : rel_pos((int64_t)off[g] + shift, base, tile);
+    }
   }
-  if (lane == 0) tseg[T] = lo;
+  if (fill_empty) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nseg; g += stride)
+      if (off[g] == off[g + 1]) out[g] = seed;
+  }
 }
 
+// One workgroup per tile.
 template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
-__device__ __forceinline__ void seg_tile(const SegArgs<T, Starts>& a, uint32_t t, uint32_t epoch,
-                                         int64_t* s_range, int64_t* s_starts, uint32_t* s_heads,
-                                         Maybe<SegV<T>>* s_wave, Maybe<SegV<T>>* s_prefix) {
+__global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
+  constexpr bool UNIFORM = std::is_same<Starts, UniformStarts>::value;
   constexpr int VEC = Vec<T>::N;
   constexpr int W = BLOCK / 64;
   constexpr int TILE = BLOCK * K * VEC;
   constexpr int NBW = TILE / 32 + 1;  // head bitmap: tile positions 0..TILE
   using S = SegV<T>;
-  constexpr int NW = Tagged<S>::NW;
+  __shared__ uint32_t s_heads[NBW];
+  __shared__ int32_t s_rel[UNIFORM ? 1 : CAP];
+  __shared__ int64_t s_range[2];
+  __shared__ S s_wave[W];
+  __shared__ int64_t s_open_g[W];
+  __shared__ T s_open_v[W];
   SegOp<Op> sop{Op{}};
   Op op{};
+  const T id = identity<T>(op);
+  const S sid{id, 0u};  // identity of the segmented combine
   const int lane = lane_id();
   const int w = threadIdx.x / 64;
+  const uint32_t t = blockIdx.x;
   const int64_t base = (int64_t)t * TILE;
 
-  // ---- segment heads of the tile.  Segment g in [0, S) covers
-  // [start(g), start(g+1)); "segment" S collects the elements after the last
-  // one and elements before start(0) belong to "segment" -1 (no results).
-  // Every start in [base, base + TILE] sets its bit in an LDS bitmap; the
-  // starts themselves are staged (when they fit) for the rare end lookups.
-  auto sv = [&](int64_t g) -> int64_t { return g > a.nseg ? INT64_MAX : a.st.start(g); };
-  if (threadIdx.x == 0) a.st.tile_range(t, base, min(base + TILE, a.e_end) - 1, a.nseg, &s_range[0], &s_range[1]);
-  for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
-  __syncthreads();
-  const int64_t g_lo = s_range[0], g_hi = s_range[1];
-  const int64_t cnt = g_hi - g_lo + 2;  // starts g_lo .. g_hi+1
-  const bool staged = cnt <= CAP;
-  for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
-    const int64_t st = sv(g_lo + i);
-    if (staged) s_starts[i] = st;
-    const int64_t p = st - base;
-    if (p >= 0 && p <= TILE) atomicOr(&s_heads[p >> 5], 1u << (p & 31));
-  }
-  __syncthreads();
-  // smallest segment whose start is at least `e`
-  auto first_seg_at = [&](int64_t e) -> int64_t {
-    int64_t x = g_lo, y = g_hi + 1;
-    while (x < y) {
-      const int64_t m = (x + y) >> 1;
-      if ((staged ? s_starts[m - g_lo] : sv(m)) < e) x = m + 1; else y = m;
+  // ---- wave 0 fetches the tile's segment range and starts first, then
+  // every wave issues its data loads; the staging below overlaps them
+  // (barriers wait only for LDS traffic)
+  TileSegs hdr{0, 0};
+  int32_t my_rel = 0;
+  if constexpr (!UNIFORM) {
+    if (w == 0) {
+      hdr = a.st.segs[t];
+      my_rel = a.st.slots[(size_t)t * SLOT + lane];
     }
-    return x;
+  }
+  // padding positions hold the identity and start a dead segment
+  T v[K][VEC];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t e0 = base + ((w * K + k) * 64 + lane) * VEC;
+    if (e0 >= a.e_lo && e0 + VEC <= a.e_end) {
+      const Vec<T> r = load16_stream(a.in + e0);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v[k][j] = r.e[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v[k][j] = (e0 + j >= a.e_lo && e0 + j < a.e_end) ? a.in[e0 + j] : id;
+    }
+  }
+
+  // ---- segment heads: every start in [base, base + TILE] sets its bit.
+  // Segment g in [0, S) covers [start(g), start(g+1)); "segment" S collects
+  // the elements after the last one and elements before start(0) belong to
+  // "segment" -1 (no results).
+  int64_t g_lo, cnt;
+  if constexpr (UNIFORM) {
+    for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
+    g_lo = max(base - a.st.shift, (int64_t)0) / a.st.len;
+    const int64_t g_hi = min(max(base + TILE - a.st.shift, (int64_t)0) / a.st.len, a.nseg);
+    cnt = g_hi - g_lo + 2;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
+      const int64_t p = a.st.start(g_lo + i) - base;
+      if (g_lo + i <= a.nseg && p >= 0 && p <= TILE) atomicOr(&s_heads[p >> 5], 1u << (p & 31));
+    }
+    __syncthreads();
+  } else {
+    if (w == 0) {
+      for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
+      if (hdr.cnt <= SLOT && lane < hdr.cnt) {
+        s_rel[lane] = my_rel;
+        if (my_rel >= 0 && my_rel <= TILE) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
+      }
+      if (lane == 0) {
+        s_range[0] = hdr.g_lo;
+        s_range[1] = hdr.cnt;
+      }
+    }
+    __syncthreads();
+    g_lo = s_range[0];
+    cnt = s_range[1];
+    if (cnt > SLOT) {
+      // many segments start in this tile: stage them from the offsets
+      for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
+        const int64_t g = g_lo + i;
+        const int32_t r = g > a.nseg ? TILE + 1 : rel_pos(a.st.start(g), base, TILE);
+        if (cnt <= CAP) s_rel[i] = r;
+        if (r >= 0 && r <= TILE) atomicOr(&s_heads[r >> 5], 1u << (r & 31));
+      }
+      __syncthreads();
+    }
+  }
+  // the segment whose last element is at tile position p
+  auto seg_ending_at = [&](int p) -> int64_t {
+    if constexpr (UNIFORM) {
+      return (base + p - a.st.shift) / a.st.len;
+    } else {
+      // smallest index i with start(g_lo + i) > p, minus one
+      int64_t x = 0, y = cnt;
+      while (x < y) {
+        const int64_t m = (x + y) >> 1;
+        const int64_t sm = cnt <= CAP ? (int64_t)s_rel[m]
+                                      : (g_lo + m > a.nseg ? (int64_t)TILE + 1 : a.st.start(g_lo + m) - base);
+        if (sm <= p) x = m + 1; else y = m;
+      }
+      return g_lo + x - 1;
+    }
   };
 
-  // ---- load; segmented scan inside each thread's dwordx4 vector
-  T xv[K][VEC];
-  uint32_t hb[K], eb[K];  // per vector: head bits of its elements, "next is a head" bits
+  // ---- per chunk: segmented scan inside each lane's vector, across the
+  // wave, and the results at segment ends (the element before a head).  An
+  // end whose segment began before the wave's portion of the tile ("open")
+  // is stashed per wave, at most one per wave.
+  if (lane == 0) s_open_g[w] = -1;  // each wave its own slot, before its own stash
+  S run = sid;  // segmented combination of the wave's earlier chunks
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int p0 = ((w * K + k) * 64 + lane) * VEC;
     const int64_t e0 = base + p0;
-    T v[VEC];
     uint32_t invalid = 0;
-    if (e0 + VEC <= a.e_end) {
-      Vec<T> r = load16_stream(a.in + e0);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) v[j] = r.e[j];
-    } else {
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const bool ok = e0 + j < a.e_end;
-        v[j] = ok ? a.in[e0 + j] : T();
-        invalid |= ok ? 0u : 1u << j;
-      }
-    }
+    for (int j = 0; j < VEC; ++j) invalid |= (e0 + j >= a.e_lo && e0 + j < a.e_end ? 0u : 1u) << j;
     const int sh = p0 & 31;
     uint64_t word = s_heads[p0 >> 5];
     if (sh + VEC + 1 > 32) word |= (uint64_t)s_heads[(p0 >> 5) + 1] << 32;
     const uint32_t b = (uint32_t)(word >> sh) & ((1u << (VEC + 1)) - 1u);
-    hb[k] = (b & ((1u << VEC) - 1u)) | invalid;  // elements past the row start a dead segment
-    eb[k] = (b >> 1) & ~invalid;
-    xv[k][0] = v[0];
+    const uint32_t hb = (b & ((1u << VEC) - 1u)) | invalid;  // heads (padding: dead segments)
+    uint32_t eb = (b >> 1) & ~invalid;                       // ends: the next position is a head
+    T x[VEC];
+    x[0] = v[k][0];
 #pragma unroll
-    for (int j = 1; j < VEC; ++j) xv[k][j] = (hb[k] >> j & 1u) ? v[j] : op(xv[k][j - 1], v[j]);
-  }
-
-  // ---- segmented scan across lanes, chunks and waves
-  auto none = []() { Maybe<S> m; m.v.v = T(); m.v.f = 0; m.ok = 0; return m; };
-  Maybe<S> pre[K];
-  Maybe<S> run = none();
+    for (int j = 1; j < VEC; ++j) x[j] = (hb >> j & 1u) ? v[k][j] : op(x[j - 1], v[k][j]);
+    const S inc = wave_inclusive_scan(S{x[VEC - 1], hb ? 1u : 0u}, sop);
+    const S up = shift_up1(inc);
+    const S P = sop(run, lane > 0 ? up : sid);  // prefix of the wave's portion before this vector
+    run = sop(run, read_lane(inc, 63));
+    while (eb) {
+      const int j = __builtin_ctz(eb);
+      eb &= eb - 1u;
+      T xj = x[0];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    S tot_l;
-    tot_l.v = xv[k][VEC - 1];
-    tot_l.f = hb[k] ? 1u : 0u;
-    S inc = wave_inclusive_scan(tot_l, sop);
-    S up = shift_up1(inc);
-    Maybe<S> exl = none();
-    if (lane > 0) { exl.v = up; exl.ok = 1; }
-    pre[k] = mcombine(sop, run, exl);
-    Maybe<S> tot{read_lane(inc, 63), 1u};
-    run = mcombine(sop, run, tot);
-  }
-  if (lane == 0) s_wave[w] = run;
-  __syncthreads();
-  Maybe<S> wp = none(), agg = none();
-#pragma unroll
-  for (int i = 0; i < W; ++i) {
-    const Maybe<S> v = s_wave[i];
-    if (i < w) wp = mcombine(sop, wp, v);
-    agg = mcombine(sop, agg, v);
-  }
-
-  // ---- carry-in from predecessor tiles
-  if (w == 0) {
-    Maybe<S> X = none();
-    uint64_t* slot = a.status + (size_t)t * NW;
-    const bool last = t + 1 == a.ntiles;
-    if (t == 0 || agg.v.f) {
-      // a head inside the tile makes its inclusive prefix final at once
-      if (lane == 0 && !last) Tagged<S>::publish(slot, agg.v, make_tag(epoch, ST_PREFIX));
-    } else if (lane == 0 && !last) {
-      Tagged<S>::publish(slot, agg.v, make_tag(epoch, ST_AGGREGATE));
-    }
-    // the first element needs a carry unless it starts a segment
-    if (t > 0 && !(s_heads[0] & 1u)) {
-      X.v = wave_lookback<S>(a.status, t, epoch, sop);
-      X.ok = 1;
-      if (!agg.v.f && lane == 0 && !last)
-        Tagged<S>::publish(slot, sop(X.v, agg.v), make_tag(epoch, ST_PREFIX));
-    }
-    if (lane == 0) *s_prefix = X;
-  }
-  __syncthreads();
-  const Maybe<S> XW = mcombine(sop, *s_prefix, wp);
-
-  // ---- results at segment ends (the element before a head)
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (eb[k]) {
-      // segmented prefix of everything before this vector
-      const Maybe<S> P = mcombine(sop, XW, pre[k]);
-      const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        if (eb[k] >> j & 1u) {
-          const bool headed = (hb[k] & ((2u << j) - 1u)) != 0;  // a head at or before j in this vector
-          T r = (headed || !P.ok) ? xv[k][j] : op(P.v.v, xv[k][j]);
-          const int64_t g = first_seg_at(e0 + j + 1) - 1;
-          if (g >= 0 && g < a.nseg) {
-            if (a.has_seed) r = op(a.seed, r);  // seed on the left
-            a.out[g] = r;
-          }
-        }
+      for (int jj = 1; jj < VEC; ++jj) xj = j == jj ? x[jj] : xj;
+      const bool headed = (hb & ((2u << j) - 1u)) != 0;  // a head at or before j in this vector
+      T r = headed ? xj : op(P.v, xj);
+      const int64_t g = seg_ending_at(p0 + j);
+      if (!headed && !P.f) {
+        s_open_g[w] = g;
+        s_open_v[w] = r;
+      } else if (g >= 0 && g < a.nseg) {
+        if (a.has_seed) r = op(a.seed, r);  // seed on the left
+        a.out[g] = r;
       }
     }
   }
+  if (lane == 0) s_wave[w] = run;
+  __syncthreads();
 
-  // ---- empty segments (their value is the seed)
-  if (a.check_empty && a.has_seed) {
-    const int64_t per = (a.nseg + a.ntiles - 1) / a.ntiles;
-    const int64_t s0 = (int64_t)t * per, s1 = min(s0 + per, a.nseg);
-    for (int64_t g = s0 + threadIdx.x; g < s1; g += BLOCK)
-      if (a.st.start(g) == a.st.start(g + 1)) a.out[g] = a.seed;
+  // ---- wave 0: open ends of later waves that a head earlier in the tile
+  // completes; the tile's own open end and aggregate go to the fix-up record
+  if (w == 0) {
+    S agg = sid;
+#pragma unroll
+    for (int i = 0; i < W; ++i) agg = sop(agg, s_wave[i]);
+    int64_t tile_g = -1;
+    T tile_v = id;
+    if (lane < W) {
+      const int64_t g = s_open_g[lane];
+      if (g >= 0) {
+        S wp = sid;
+#pragma unroll
+        for (int i = 0; i < W; ++i)
+          if (i < lane) wp = sop(wp, s_wave[i]);
+        T r = op(wp.v, s_open_v[lane]);
+        if (wp.f) {
+          if (g < a.nseg) {
+            if (a.has_seed) r = op(a.seed, r);
+            a.out[g] = r;
+          }
+        } else {
+          tile_g = g;  // no head before it in the tile: the tile's open end
+          tile_v = r;
+        }
+      }
+    }
+    // at most one lane holds the tile's open end
+    const uint64_t m = __ballot(tile_g >= 0);
+    if (m) {
+      const int src = __builtin_ctzll(m);
+      tile_g = read_lane(tile_g, src);
+      tile_v = read_lane(tile_v, src);
+    }
+    if (lane == 0) {
+      TileCarry<T> c;
+      c.agg = agg;
+      c.open_v = tile_v;
+      c.open_g = m ? tile_g : -1;
+      a.carry[t] = c;
+    }
   }
 }
 
-// Persistent workgroups taking tiles by ticket (next ticket prefetched).
-template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
-__global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
-  constexpr int TILE = BLOCK * K * Vec<T>::N;
-  __shared__ uint64_t s_tick[2];
-  __shared__ int64_t s_range[2];
-  __shared__ int64_t s_starts[CAP];
-  __shared__ uint32_t s_heads[TILE / 32 + 1];
-  __shared__ Maybe<SegV<T>> s_wave[BLOCK / 64];
-  __shared__ Maybe<SegV<T>> s_prefix;
-  if (threadIdx.x == 0) s_tick[0] = take_ticket(a.ctl);
+// One workgroup: segmented exclusive scan over the tile aggregates gives each
+// tile the carry of the segment it starts in; that completes the open ends.
+template <typename T, typename Op, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_fix(const TileCarry<T>* carry, uint32_t ntiles, T* out,
+                                                   int64_t nseg, uint32_t has_seed, T seed) {
+  using S = SegV<T>;
+  __shared__ S s_part[BLOCK / 64];
+  SegOp<Op> sop{Op{}};
+  Op op{};
+  const S sid{identity<T>(op), 0u};
+  const int lane = lane_id(), w = threadIdx.x / 64;
+  const uint32_t per = (ntiles + BLOCK - 1) / BLOCK;
+  const uint32_t t0 = min(threadIdx.x * per, ntiles), t1 = min(t0 + per, ntiles);
+  S own = sid;
+  for (uint32_t t = t0; t < t1; ++t) own = sop(own, carry[t].agg);
+  // exclusive segmented scan of the per-thread combinations
+  const S inc = wave_inclusive_scan(own, sop);
+  const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
+  S X = lane > 0 ? up : sid;
+  if (lane == 63) s_part[w] = inc;
   __syncthreads();
-  const uint64_t tk0 = s_tick[0];
-  const uint32_t epoch = (uint32_t)(tk0 >> 32);
-  uint32_t t = (uint32_t)tk0;
-  int cur = 0;
-  while (t < a.ntiles) {
-    if (threadIdx.x == 0) s_tick[cur ^ 1] = take_ticket(a.ctl);
-    seg_tile<T, Op, Starts, BLOCK, K, CAP>(a, t, epoch, s_range, s_starts, s_heads, s_wave, &s_prefix);
-    __syncthreads();
-    cur ^= 1;
-    t = (uint32_t)s_tick[cur];
+  S wp = sid;
+#pragma unroll
+  for (int i = 0; i < BLOCK / 64; ++i)
+    if (i < w) wp = sop(wp, s_part[i]);
+  X = sop(wp, X);
+  for (uint32_t t = t0; t < t1; ++t) {
+    const TileCarry<T> c = carry[t];
+    if (c.open_g >= 0 && c.open_g < nseg) {
+      T r = op(X.v, c.open_v);
+      if (has_seed) r = op(seed, r);
+      out[c.open_g] = r;
+    }
+    X = sop(X, c.agg);
   }
-  block_retire(a.ctl, gridDim.x, epoch);
 }
 
 #ifndef ACC_SEG_K
-#define ACC_SEG_K 8
+#define ACC_SEG_K 16
 #endif
 template <typename T> struct SegTune {
   static constexpr int BLOCK = 256;
   static constexpr int K = ACC_SEG_K;  // dwordx4 vectors per thread
-  static constexpr int CAP = 1024;
+  static constexpr int CAP = 1024;     // starts staged in LDS (beyond: searched in HBM)
 };
 
 template <typename T, typename Op, typename Starts>
@@ -304,41 +409,52 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
                     const T* seed, bool check_empty, const void* offsets, size_t offset_bytes) {
   constexpr int BLOCK = SegTune<T>::BLOCK, K = SegTune<T>::K, CAP = SegTune<T>::CAP;
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
-  constexpr int NW = Tagged<SegV<T>>::NW;
-  if ((uintptr_t)in % 16) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: input must be 16-byte aligned");
-  const int64_t ntiles = ceil_div(e_end, TILE);
+  if ((uintptr_t)in % sizeof(T)) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: input not element-aligned");
+  // positions count from the 16-byte boundary at or below `in`
+  const int64_t shift = (int64_t)(((uintptr_t)in % 16) / sizeof(T));
+  const T* in0 = in - shift;
+  st.shift = shift;
+  const int64_t ntiles = e_end == 0 ? 0 : ceil_div(e_end + shift, TILE);
   if (ntiles == 0) {
     // no elements: every segment is empty
     if (!seed || !check_empty) return ACC_SUCCESS;
     return acc_hip_fill_async(s, out, seed, sizeof(T), (size_t)nseg);
   }
   if (ntiles >= 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: too many elements");
+  // workspace: per-tile carry records, then (offsets) segment ranges and starts
   SyncCtl* ctl;
-  uint64_t* status;
-  const size_t status_bytes = ((size_t)ntiles * NW * 8 + 255) / 256 * 256;
-  const size_t tseg_bytes = offsets ? (size_t)(ntiles + 1) * 8 : 0;
-  ACC_CHECK(stream_workspace(s, status_bytes + tseg_bytes, &ctl, &status));
+  uint64_t* ws;
+  const size_t carry_bytes = ((size_t)ntiles * sizeof(TileCarry<T>) + 255) / 256 * 256;
+  const size_t segs_bytes = offsets ? ((size_t)ntiles * sizeof(TileSegs) + 255) / 256 * 256 : 0;
+  const size_t slot_bytes = offsets ? (size_t)ntiles * SLOT * sizeof(int32_t) : 0;
+  ACC_CHECK(stream_workspace(s, carry_bytes + segs_bytes + slot_bytes, &ctl, &ws));
+  TileCarry<T>* carry = (TileCarry<T>*)ws;
+  const bool fill_empty = seed && check_empty;
   if constexpr (!std::is_same<Starts, UniformStarts>::value) {
-    int64_t* tseg = (int64_t*)((char*)status + status_bytes);
-    const unsigned pg = (unsigned)((ntiles + 1 + 3) / 4);
+    TileSegs* segs = (TileSegs*)((char*)ws + carry_bytes);
+    int32_t* slots = (int32_t*)((char*)ws + carry_bytes + segs_bytes);
+    const unsigned pg = (unsigned)((ntiles + 3) / 4);
+    const T sd = seed ? *seed : T();
     if (offset_bytes == 4)
-      hipLaunchKernelGGL(k_tile_seg<int32_t>, dim3(pg), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
-                         TILE, (uint32_t)ntiles, tseg);
+      hipLaunchKernelGGL((k_tile_seg<T, int32_t>), dim3(pg), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
+                         TILE, (uint32_t)ntiles, shift, segs, slots, out, fill_empty ? 1u : 0u, sd);
     else
-      hipLaunchKernelGGL(k_tile_seg<int64_t>, dim3(pg), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
-                         TILE, (uint32_t)ntiles, tseg);
+      hipLaunchKernelGGL((k_tile_seg<T, int64_t>), dim3(pg), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
+                         TILE, (uint32_t)ntiles, shift, segs, slots, out, fill_empty ? 1u : 0u, sd);
     ACC_HIP_CHECK(hipGetLastError());
-    st.tseg = tseg;
+    st.segs = segs;
+    st.slots = slots;
   }
-  SegArgs<T, Starts> a{in, out, st, nseg, e_end, (uint32_t)ntiles, seed ? 1u : 0u, seed ? *seed : T(),
-                       check_empty ? 1u : 0u, ctl, status};
-  auto kern = k_fold_seg<T, Op, Starts, BLOCK, K, CAP>;
-  const DeviceInfo* di;
-  ACC_CHECK(device_info(s->device, &di));
-  int64_t grid = (int64_t)di->compute_units * blocks_per_cu((const void*)kern, BLOCK);
-  if (grid > ntiles) grid = ntiles;
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BLOCK), 0, s->stream, a);
+  SegArgs<T, Starts> a{in0, out, st, nseg, shift, e_end + shift, (uint32_t)ntiles, seed ? 1u : 0u,
+                       seed ? *seed : T(), carry};
+  hipLaunchKernelGGL((k_fold_seg<T, Op, Starts, BLOCK, K, CAP>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s->stream,
+                     a);
   ACC_HIP_CHECK(hipGetLastError());
+  if (ntiles > 1) {
+    hipLaunchKernelGGL((k_seg_fix<T, Op, 1024>), dim3(1), dim3(1024), 0, s->stream, (const TileCarry<T>*)carry,
+                       (uint32_t)ntiles, out, nseg, seed ? 1u : 0u, seed ? *seed : T());
+    ACC_HIP_CHECK(hipGetLastError());
+  }
   return ACC_SUCCESS;
 }
 
@@ -348,7 +464,7 @@ int fold_dim(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void
     using T = decltype(z);
     return with_op<T>(op, [&](auto f) {
       using Op = decltype(f);
-      return fold_seg_launch<T, Op, UniformStarts>(s, (const T*)in, (T*)out, UniformStarts{inner}, outer,
+      return fold_seg_launch<T, Op, UniformStarts>(s, (const T*)in, (T*)out, UniformStarts{inner, 0}, outer,
                                                    outer * inner, (const T*)seed, false, nullptr, 0);
     });
   });
@@ -381,11 +497,11 @@ extern "C" int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const voi
         int rc;
         if (I == 4)
           rc = fold_seg_launch<T, Op, OffsetStarts<int32_t>>(
-              s, rin, rout, OffsetStarts<int32_t>{(const int32_t*)offsets, nullptr}, nseg, inner,
+              s, rin, rout, OffsetStarts<int32_t>{(const int32_t*)offsets, nullptr, nullptr, 0}, nseg, inner,
               (const T*)seed, true, offsets, 4);
         else
           rc = fold_seg_launch<T, Op, OffsetStarts<int64_t>>(
-              s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr}, nseg, inner,
+              s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr, nullptr, 0}, nseg, inner,
               (const T*)seed, true, offsets, 8);
         if (rc) return rc;
       }

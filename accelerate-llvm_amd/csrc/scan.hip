@@ -24,80 +24,122 @@ namespace acc {
 
 template <typename T> struct ScanArgs {
   TileIO<T> io;
-  uint32_t ntiles;
+  uint32_t ntiles;  // all tiles of the scan, the ragged one included
+  uint32_t t0;      // status index of the first tile this launch processes
+  uint32_t nfull;   // tiles this launch processes (all full)
   SyncCtl* ctl;
 };
 
-// Persistent workgroups: each takes tiles by ticket until none are left,
-// prefetching the next ticket while it scans the current tile.
+// Persistent workgroups over the full tiles: each takes a tile by ticket when
+// it is ready to start it, so tickets are started in order and a tile's
+// predecessors have started no later than it did.  (Taking the next ticket
+// ahead of time lets consecutive tiles start up to a whole tile apart, and
+// every look-back then waits for that skew.)
 template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED>
 __global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs<T> a) {
   constexpr int VEC = Vec<T>::N;
   constexpr int TILE = BLOCK * K * VEC;
-  __shared__ uint64_t s_tick[2];
-  __shared__ Maybe<T> s_wave[BLOCK / 64];
-  __shared__ Maybe<T> s_prefix;
+  __shared__ uint64_t s_tick;
+  __shared__ T s_wave[BLOCK / 64];
+  __shared__ T s_prefix;
   using OpT = typename std::conditional<REVERSE, Flip<Op>, Op>::type;
   OpT op{};
-  if (threadIdx.x == 0) s_tick[0] = take_ticket(a.ctl);
+  if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
   __syncthreads();
-  const uint64_t tk0 = s_tick[0];
+  const uint64_t tk0 = s_tick;
   const uint32_t epoch = (uint32_t)(tk0 >> 32);
-  uint32_t t = (uint32_t)tk0;
-  int cur = 0;
-  while (t < a.ntiles) {
-    if (threadIdx.x == 0) s_tick[cur ^ 1] = take_ticket(a.ctl);
+  uint32_t i = (uint32_t)tk0;
+  while (i < a.nfull) {
+    const uint32_t t = a.t0 + i;
     const int64_t p = REVERSE ? (int64_t)(a.ntiles - 1 - t) : (int64_t)t;
-    const int64_t base = p * TILE;
-    if (base + TILE <= a.io.n)
-      scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, false>(a.io, op, t, base, epoch, s_wave, &s_prefix);
-    else
-      scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, true>(a.io, op, t, base, epoch, s_wave, &s_prefix);
+    scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, false>(a.io, op, t, p * TILE, epoch, s_wave, &s_prefix);
     __syncthreads();
-    cur ^= 1;
-    t = (uint32_t)s_tick[cur];
+    if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
+    __syncthreads();
+    i = (uint32_t)s_tick;
   }
   block_retire(a.ctl, gridDim.x, epoch);
 }
 
+// The ragged tile (the last of a left-to-right scan, the first of a
+// right-to-left one) in a one-workgroup launch of its own, so that its masked
+// accesses do not raise the register budget of the persistent kernel.  It
+// runs before k_scan when it is the first tile (publishing its prefix under
+// the epoch k_scan will use) and after k_scan when it is the last (whose
+// epoch k_scan has just advanced: epoch_back = 1).
+template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED>
+__global__ __launch_bounds__(BLOCK) void k_scan_ragged(ScanArgs<T> a, uint32_t t, uint32_t epoch_back) {
+  __shared__ uint32_t s_epoch;
+  __shared__ T s_wave[BLOCK / 64];
+  __shared__ T s_prefix;
+  using OpT = typename std::conditional<REVERSE, Flip<Op>, Op>::type;
+  const uint32_t epoch = block_epoch(a.ctl, &s_epoch) - epoch_back;
+  const int64_t p = REVERSE ? (int64_t)(a.ntiles - 1 - t) : (int64_t)t;
+  scan_tile<T, OpT, BLOCK, K, REVERSE, ALIGNED, true>(a.io, OpT{}, t, p * (int64_t)(BLOCK * K * Vec<T>::N), epoch,
+                                                      s_wave, &s_prefix);
+}
+
 #ifndef ACC_SCAN_K
-#define ACC_SCAN_K 8
+#define ACC_SCAN_K 16
+#endif
+#ifndef ACC_SCAN_BLOCK
+#define ACC_SCAN_BLOCK 256
 #endif
 template <typename T> struct ScanTune {
-  static constexpr int BLOCK = 256;
-  static constexpr int K = ACC_SCAN_K;  // dwordx4 vectors per thread
+  static constexpr int BLOCK = ACC_SCAN_BLOCK;
+  static constexpr int K = ACC_SCAN_K;   // dwordx4 vectors per thread
+  static constexpr int K_UNALIGNED = 4;  // element accesses: a smaller tile
 };
 
 template <typename T, typename Op, bool REVERSE, bool ALIGNED>
-int scan_launch(acc_stream_t s, const ScanArgs<T>& a) {
-  constexpr int BLOCK = ScanTune<T>::BLOCK, K = ScanTune<T>::K;
-  auto kern = k_scan<T, Op, BLOCK, K, REVERSE, ALIGNED>;
-  const DeviceInfo* di;
-  ACC_CHECK(device_info(s->device, &di));
-  int64_t grid = (int64_t)di->compute_units * blocks_per_cu((const void*)kern, BLOCK);
-  if (grid > (int64_t)a.ntiles) grid = a.ntiles;
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BLOCK), 0, s->stream, a);
-  ACC_HIP_CHECK(hipGetLastError());
+int scan_launch(acc_stream_t s, ScanArgs<T> a) {
+  constexpr int BLOCK = ScanTune<T>::BLOCK, K = ALIGNED ? ScanTune<T>::K : ScanTune<T>::K_UNALIGNED;
+  constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
+  const bool ragged = a.io.n % TILE != 0;
+  const uint32_t nfull = a.ntiles - (ragged ? 1u : 0u);
+  // the ragged tile's traversal index: the last tile, or the first when reversed
+  const uint32_t tr = REVERSE ? 0u : a.ntiles - 1;
+  if (ragged && REVERSE) {
+    hipLaunchKernelGGL((k_scan_ragged<T, Op, BLOCK, K, REVERSE, ALIGNED>), dim3(1), dim3(BLOCK), 0, s->stream, a,
+                       tr, 0u);
+    ACC_HIP_CHECK(hipGetLastError());
+  }
+  if (nfull > 0) {
+    auto kern = k_scan<T, Op, BLOCK, K, REVERSE, ALIGNED>;
+    const DeviceInfo* di;
+    ACC_CHECK(device_info(s->device, &di));
+    int64_t grid = (int64_t)di->compute_units * blocks_per_cu((const void*)kern, BLOCK);
+    if (grid > (int64_t)nfull) grid = nfull;
+    a.t0 = ragged && REVERSE ? 1u : 0u;
+    a.nfull = nfull;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BLOCK), 0, s->stream, a);
+    ACC_HIP_CHECK(hipGetLastError());
+  }
+  if (ragged && !REVERSE) {
+    hipLaunchKernelGGL((k_scan_ragged<T, Op, BLOCK, K, REVERSE, ALIGNED>), dim3(1), dim3(BLOCK), 0, s->stream, a,
+                       tr, nfull > 0 ? 1u : 0u);
+    ACC_HIP_CHECK(hipGetLastError());
+  }
   return ACC_SUCCESS;
 }
 
 template <typename T, typename Op>
 int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const T* in, int64_t n,
              T* out, T* total_out, T* seed_out) {
-  constexpr int BLOCK = ScanTune<T>::BLOCK, K = ScanTune<T>::K, VEC = Vec<T>::N;
-  constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
+  constexpr int BLOCK = ScanTune<T>::BLOCK, VEC = Vec<T>::N;
   constexpr int NW = Tagged<T>::NW;
-  const int64_t ntiles = ceil_div(n, TILE);
-  if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "scan: %lld elements", (long long)n);
   if ((uintptr_t)in % sizeof(T) || (uintptr_t)out % sizeof(T))
     return fail(ACC_ERROR_INVALID_VALUE, "scan: arrays not element-aligned");
   const bool aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+  const int64_t TILE = (int64_t)BLOCK * (aligned ? ScanTune<T>::K : ScanTune<T>::K_UNALIGNED) * VEC;
+  const int64_t ntiles = ceil_div(n, TILE);
+  if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "scan: %lld elements", (long long)n);
   SyncCtl* ctl;
   uint64_t* status;
   ACC_CHECK(stream_workspace(s, (size_t)ntiles * NW * 8, &ctl, &status));
   TileIO<T> io{in, out, n, 0u, (uint32_t)(ntiles - 1), exclusive ? 1u : 0u, seed ? 1u : 0u, seed ? *seed : T(),
                total_out, seed_out, status};
-  ScanArgs<T> a{io, (uint32_t)ntiles, ctl};
+  ScanArgs<T> a{io, (uint32_t)ntiles, 0u, 0u, ctl};
   if (reverse) return aligned ? scan_launch<T, Op, true, true>(s, a) : scan_launch<T, Op, true, false>(s, a);
   return aligned ? scan_launch<T, Op, false, true>(s, a) : scan_launch<T, Op, false, false>(s, a);
 }

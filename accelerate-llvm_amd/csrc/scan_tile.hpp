@@ -25,26 +25,6 @@ template <typename T> struct TileIO {
   uint64_t* status;
 };
 
-template <typename T, typename Op, bool MASKED> struct Lane;
-
-// Element storage without masks: plain values.
-template <typename T, typename Op> struct Lane<T, Op, false> {
-  using E = T;
-  static __device__ __forceinline__ E make(T v, bool) { return v; }
-  static __device__ __forceinline__ E cmb(Op op, E a, E b) { return op(a, b); }
-  static __device__ __forceinline__ Maybe<T> lift(E e) { return Maybe<T>{e, 1u}; }
-  static __device__ __forceinline__ E wscan(E x, Op op) { return wave_inclusive_scan(x, op); }
-};
-
-// Element storage in the ragged tile: optional values.
-template <typename T, typename Op> struct Lane<T, Op, true> {
-  using E = Maybe<T>;
-  static __device__ __forceinline__ E make(T v, bool ok) { return Maybe<T>{v, ok ? 1u : 0u}; }
-  static __device__ __forceinline__ E cmb(Op op, E a, E b) { return mcombine(op, a, b); }
-  static __device__ __forceinline__ Maybe<T> lift(E e) { return e; }
-  static __device__ __forceinline__ E wscan(E x, Op op) { return wave_inclusive_scan_m(x, op); }
-};
-
 template <typename T> __device__ __forceinline__ Maybe<T> none() {
   Maybe<T> m;
   m.v = T();
@@ -52,14 +32,19 @@ template <typename T> __device__ __forceinline__ Maybe<T> none() {
   return m;
 }
 
+// Scan of one tile of BLOCK*K*VEC elements.  Each lane holds K dwordx4
+// vectors; vector k of wave w is vector (w*K + k)*64 + lane of the tile, so
+// every load instruction of a wave reads 1 KiB of consecutive bytes.  The
+// ragged tile (MASKED) is padded with the combine's identity, which leaves
+// every valid prefix, the aggregate and the total exactly as without padding;
+// only its one partial vector uses element accesses.
 template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED, bool MASKED>
 __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t, int64_t base,
-                                          uint32_t epoch, Maybe<T>* s_wave, Maybe<T>* s_prefix) {
+                                          uint32_t epoch, T* s_wave, T* s_prefix) {
   constexpr int VEC = Vec<T>::N;
   constexpr int W = BLOCK / 64;
   constexpr int NW = Tagged<T>::NW;
-  using L = Lane<T, Op, MASKED>;
-  using E = typename L::E;
+  const T id = identity<T>(op);
   const int lane = lane_id();
   const int w = threadIdx.x / 64;
 
@@ -69,115 +54,107 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
                    : ((int64_t)(w * K + k) * 64 + lane);
   };
 
-  // ---- load, thread-level inclusive scan of each vector
-  E x[K][VEC];
+  // ---- ragged tile: bit k*VEC + j says physical element j of vector k is
+  // inside the array (one 64-bit mask per lane instead of a lane mask per
+  // element)
+  static_assert(K * VEC <= 64, "validity mask");
+  constexpr uint32_t VMASK = (1u << VEC) - 1u;
+  uint64_t valid = ~0ull;
+  if constexpr (MASKED) {
+    valid = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t r = a.n - (base + vidx(k) * VEC);
+      const int nv = r <= 0 ? 0 : (r >= VEC ? VEC : (int)r);
+      valid |= (uint64_t)((1u << nv) - 1u) << (k * VEC);
+    }
+  }
+  auto ok = [&](int k, int j) -> bool { return !MASKED || ((valid >> (k * VEC + j)) & 1u); };
+
+  // ---- load, thread-level inclusive scan of each vector (traversal order)
+  T x[K][VEC];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t e0 = base + vidx(k) * VEC;
     T v[VEC];
-    bool ok[VEC];
-    if (ALIGNED && !MASKED) {
-      Vec<T> r = load16_stream(a.in + e0);
+    if (ALIGNED) {
+      // In the ragged tile a vector past the end re-reads the vector holding
+      // the last element (an aligned 16-byte block with a valid element never
+      // crosses a page) and every element past the end becomes the identity.
+      const int64_t ea = MASKED ? min(e0, ((a.n - 1) / VEC) * VEC) : e0;
+      const Vec<T> r = load16_stream(a.in + ea);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) { v[j] = r.e[j]; ok[j] = true; }
+      for (int j = 0; j < VEC; ++j) v[j] = ok(k, j) ? r.e[j] : id;
     } else {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const int64_t i = e0 + j;
-        ok[j] = !MASKED || i < a.n;
-        v[j] = ok[j] ? a.in[i] : T();
-      }
+      for (int j = 0; j < VEC; ++j) v[j] = ok(k, j) ? a.in[e0 + j] : id;
     }
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) {
-      const int jj = REVERSE ? VEC - 1 - j : j;
-      x[k][j] = L::make(v[jj], ok[jj]);
-    }
+    for (int j = 0; j < VEC; ++j) x[k][j] = v[REVERSE ? VEC - 1 - j : j];
 #pragma unroll
-    for (int j = 1; j < VEC; ++j) x[k][j] = L::cmb(op, x[k][j - 1], x[k][j]);
+    for (int j = 1; j < VEC; ++j) x[k][j] = op(x[k][j - 1], x[k][j]);
   }
 
   // ---- wave scan of lane totals, chunk by chunk
-  Maybe<T> pre[K];
-  Maybe<T> run = none<T>();
+  T pre[K];
+  T run = id;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    E inc = L::wscan(x[k][VEC - 1], op);
-    E up = shift_up1(inc);
-    Maybe<T> exl = lane > 0 ? L::lift(up) : none<T>();
-    pre[k] = mcombine(op, run, exl);
-    run = mcombine(op, run, L::lift(read_lane(inc, 63)));
+    const T inc = wave_inclusive_scan(x[k][VEC - 1], op);
+    const T up = shift_up1(inc);
+    pre[k] = op(run, lane > 0 ? up : id);
+    run = op(run, read_lane(inc, 63));
   }
   if (lane == 0) s_wave[w] = run;
   __syncthreads();
-  Maybe<T> wp = none<T>(), agg = none<T>();
+  T wp = id, agg = id;
 #pragma unroll
   for (int i = 0; i < W; ++i) {
-    const Maybe<T> v = s_wave[i];
-    if (i < w) wp = mcombine(op, wp, v);
-    agg = mcombine(op, agg, v);
+    const T v = s_wave[i];
+    if (i < w) wp = op(wp, v);
+    agg = op(agg, v);
   }
 
   // ---- tile prefix: seed for the first tile, look-back otherwise
   if (w == 0) {
-    Maybe<T> X = none<T>();
+    T X = id;
     uint64_t* slot = a.status + (size_t)t * NW;
     if (t == a.first) {
-      if (a.has_seed) X = Maybe<T>{a.seed, 1u};
+      if (a.has_seed) X = a.seed;
     } else {
-      if (lane == 0) Tagged<T>::publish(slot, agg.v, make_tag(epoch, ST_AGGREGATE));
-      X = Maybe<T>{wave_lookback<T>(a.status, t, epoch, op, a.first), 1u};
+      if (lane == 0) Tagged<T>::publish(slot, agg, make_tag(epoch, ST_AGGREGATE));
+      X = wave_lookback<T>(a.status, t, epoch, op, a.first);
     }
-    const Maybe<T> incl = mcombine(op, X, agg);
+    const T incl = op(X, agg);
     if (lane == 0) {
-      if (t != a.last) Tagged<T>::publish(slot, incl.v, make_tag(epoch, ST_PREFIX));
+      if (t != a.last) Tagged<T>::publish(slot, incl, make_tag(epoch, ST_PREFIX));
       *s_prefix = X;
-      if (t == a.last && a.total_out) *a.total_out = incl.v;
+      if (t == a.last && a.total_out) *a.total_out = incl;
       if (t == a.first && a.seed_out) *a.seed_out = a.seed;
     }
   }
   __syncthreads();
-  const Maybe<T> X = *s_prefix;
-  const Maybe<T> XW = mcombine(op, X, wp);
+  const T XW = op(*s_prefix, wp);
 
   // ---- outputs
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const Maybe<T> P = mcombine(op, XW, pre[k]);
+    const T P = op(XW, pre[k]);
     T y[VEC];
-    bool ok[VEC];
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) {
-      const Maybe<T> xi = L::lift(x[k][j]);
-      if (a.exclusive) {
-        // exclusive value: everything before this element (always defined: a seed is given)
-        const Maybe<T> prev = j == 0 ? P : mcombine(op, P, L::lift(x[k][j - 1]));
-        y[j] = prev.v;
-      } else {
-        y[j] = mcombine(op, P, xi).v;
-      }
-      ok[j] = true;
-    }
-    if constexpr (MASKED) {
-      // validity of each element in its own (physical) position
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const int64_t i = base + vidx(k) * VEC + (REVERSE ? VEC - 1 - j : j);
-        ok[j] = i < a.n;
-      }
-    }
+    for (int j = 0; j < VEC; ++j)
+      y[j] = a.exclusive ? (j == 0 ? P : op(P, x[k][j - 1])) : op(P, x[k][j]);
     const int64_t e0 = base + vidx(k) * VEC;
-    if (ALIGNED && !MASKED) {
+    const uint32_t vk = MASKED ? (uint32_t)(valid >> (k * VEC)) & VMASK : VMASK;
+    if (ALIGNED && vk == VMASK) {
       Vec<T> r;
 #pragma unroll
       for (int j = 0; j < VEC; ++j) r.e[j] = y[REVERSE ? VEC - 1 - j : j];
       store16(a.out + e0, r);
-    } else {
+    } else if (vk) {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const int jl = REVERSE ? VEC - 1 - j : j;  // logical slot of physical element j
-        if (ok[jl]) a.out[e0 + j] = y[jl];
-      }
+      for (int j = 0; j < VEC; ++j)
+        if (vk >> j & 1u) a.out[e0 + j] = y[REVERSE ? VEC - 1 - j : j];
     }
   }
 }

@@ -33,11 +33,23 @@ def zipf_i32(seed: int, s: float, nbins: int, n: int) -> np.ndarray:
 
 def segment_lengths(seed: int, nseg: int, maxlen: int, total: int) -> np.ndarray:
     """nseg Int64 lengths, log-uniform in [1, maxlen], scaled to sum to total
-    (each at least 1; the rounding residual goes to the longest)."""
+    (each at least 1; a rounding shortfall goes to the longest, an excess is
+    taken off the longest ones, never below 1)."""
+    if nseg > total:
+        raise ValueError("more segments than elements")
     rng = np.random.default_rng(seed)
     l = np.exp(rng.uniform(0.0, np.log(maxlen), size=nseg))
     out = np.maximum(1, np.rint(l * (total / l.sum()))).astype(np.int64)
-    out[np.argmax(out)] += total - int(out.sum())
+    diff = total - int(out.sum())
+    if diff >= 0:
+        out[np.argmax(out)] += diff
+    else:
+        order = np.argsort(-out, kind="stable")
+        room = np.cumsum(out[order] - 1)  # what the longest k can give up
+        k = int(np.searchsorted(room, -diff)) + 1
+        take = out[order[:k]] - 1
+        take[-1] -= int(room[k - 1]) + diff  # the k-th gives only the rest
+        out[order[:k]] -= take
     return out
 
 

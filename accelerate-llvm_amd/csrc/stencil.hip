@@ -60,13 +60,16 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   __shared__ __align__(16) T tile[LH * LW];
 
   // XCD-aware tile order: consecutive workgroups land on different XCDs, so
-  // when the tile count allows, each XCD gets a contiguous band of tiles
-  // (vertical neighbours then share halo rows in that XCD's L2).
+  // when the tile count allows, each XCD gets a contiguous band of tiles,
+  // walked row by row.  Horizontal neighbours then run together on one XCD
+  // and share, in its L2, the 128-byte lines that hold each row's horizontal
+  // halo (a column-major walk fetched every such line twice from HBM); the
+  // vertical halo is 2*RY of 64 rows.
   const uint32_t nt = a.tiles_x * a.tiles_y;
   uint32_t b = blockIdx.x;
   if (nt % 8u == 0u) b = (b % 8u) * (nt / 8u) + b / 8u;
-  const uint32_t tx = b / a.tiles_y;  // column-major: walk down a column band
-  const uint32_t ty = b - tx * a.tiles_y;
+  const uint32_t ty = b / a.tiles_x;  // row-major
+  const uint32_t tx = b - ty * a.tiles_x;
   const int64_t x0 = (int64_t)tx * ST_TW, y0 = (int64_t)ty * ST_TH;
 
   // ---- stage the halo tile; coordinates outside the array go through the

@@ -535,6 +535,17 @@ void or_gen_segment_lengths(uint64_t seed, int64_t nseg, int64_t maxlen, int64_t
     tot += v;
     if (out[i] > out[big]) big = i;
   }
-  out[big] += total - tot;
+  if (tot <= total) {
+    out[big] += total - tot;
+  } else {
+    // rounding and the floor of 1 oversupplied: take the excess off the
+    // segments in index order, never below 1 (needs nseg <= total)
+    int64_t excess = tot - total;
+    for (int64_t i = 0; i < nseg && excess > 0; ++i) {
+      const int64_t d = out[i] - 1 < excess ? out[i] - 1 : excess;
+      out[i] -= d;
+      excess -= d;
+    }
+  }
   free(l);
 }

@@ -243,6 +243,8 @@ def gen_zipf_i32(seed, s, nbins, n):
 
 
 def gen_segment_lengths(seed, nseg, maxlen, total):
+    if nseg > total:
+        raise ValueError("more segments than elements")
     out = np.empty(nseg, np.int64)
     lib().or_gen_segment_lengths(c_uint64(seed), c_int64(nseg), c_int64(maxlen), c_int64(total), _ptr(out))
     return out

@@ -78,3 +78,15 @@ def test_workload_generators():
     f = W.uniform_f32(1, -1.0, 1.0, 1000)
     assert f.dtype == np.float32 and f.min() >= -1 and f.max() < 1
     assert W.normal_f32(3, 10).dtype == np.float32
+
+
+def test_segment_lengths_dense_stay_positive():
+    # nearly as many segments as elements: the rounding excess must not drive
+    # a length below 1 (offsets stay monotonic)
+    from acc_hip import workloads as W
+    import numpy as np
+    for nseg, maxlen, total in ((3145745, 3, 3145745), (1000, 5, 1200), (7, 1000, 7)):
+        lens = W.segment_lengths(21, nseg, maxlen, total)
+        assert lens.min() >= 1 and int(lens.sum()) == total
+    with pytest.raises(ValueError):
+        W.segment_lengths(1, 10, 3, 9)

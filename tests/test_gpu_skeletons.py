@@ -128,7 +128,8 @@ def test_fold_known_answer_runq(stream):
 
 
 # -------------------------------------------------------------------- scan
-SCAN_SIZES = [1, 2, 7, 255, 4095, 4096, 4097, 12345, 65536 + 1, (1 << 20) + 17]
+# around the 16384-element Int32 tile (and the 8192-element 8-byte tile)
+SCAN_SIZES = [1, 2, 7, 255, 4095, 4096, 4097, 12345, 16383, 16384, 16385, 32768 + 3, 65536 + 1, (1 << 20) + 17]
 
 
 def gpu_scan(stream, op, direction, seed, x, outer=1):
@@ -161,7 +162,7 @@ def test_scan_int32_all_modes(stream, n, direction):
 
 
 @pytest.mark.parametrize("dt", [np.int64, np.uint32, np.uint64, np.float64])
-@pytest.mark.parametrize("n", [5, 2049, 100001])
+@pytest.mark.parametrize("n", [5, 2049, 8191, 8192, 8193, 16385, 100001])
 @pytest.mark.parametrize("direction", [L2R, R2L])
 def test_scan_other_types(stream, dt, n, direction):
     x = rand(dt, n, 7)
@@ -304,6 +305,75 @@ def test_fold_seg_rows(stream):
     o = A.DeviceArray(16, np.int32)
     A.fold_seg(stream, ADD, np.int32(0), dev(stream, x), 4, 1024, dev(stream, off), 4, o)
     assert np.array_equal(o.to_numpy(stream), O.fold_seg(ADD, np.int32(0), x, off, outer=4))
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+@pytest.mark.parametrize("dt", [np.int32, np.float32, np.int64])
+def test_fold_seg_misaligned_view(stream, offset, dt):
+    # the input starts off a 16-byte boundary: the tile grid is aligned below it
+    off = make_offsets(300, 2000, 200000, 11)
+    n = int(off[-1])
+    x = rand(dt, n + offset, 12)
+    d = dev(stream, x)
+    o = A.DeviceArray(300, dt)
+    A.fold_seg(stream, MAX, None, d.view(offset, n), 1, n, dev(stream, off), 300, o)
+    assert np.array_equal(o.to_numpy(stream), O.fold_seg(MAX, None, x[offset:], off))
+    if dt != np.float32:
+        A.fold_seg(stream, ADD, dt(3), d.view(offset, n), 1, n, dev(stream, off), 300, o)
+        assert np.array_equal(o.to_numpy(stream), O.fold_seg(ADD, dt(3), x[offset:], off))
+
+
+def test_fold_seg_rows_unaligned(stream):
+    # rows of 1001 Int32: every row after the first starts off a 16-byte boundary
+    outer, inner = 5, 1001
+    x = rand(np.int32, outer * inner, 13)
+    off = np.array([0, 1, 1, 400, 999, 1001], np.int64)
+    o = A.DeviceArray(outer * 5, np.int32)
+    A.fold_seg(stream, ADD, np.int32(-1), dev(stream, x), outer, inner, dev(stream, off), 5, o)
+    assert np.array_equal(o.to_numpy(stream), O.fold_seg(ADD, np.int32(-1), x, off, outer=outer))
+
+
+@pytest.mark.parametrize("maxlen", [3, 40, 300, 70000])
+@pytest.mark.parametrize("itype", [np.int32, np.int64])
+def test_fold_seg_starts_per_tile(stream, maxlen, itype):
+    # ~8000, ~400, ~50 and < 1 segment starts per 8192-element Int64 tile: the
+    # staged-in-HBM, staged-in-LDS, handed-over and carry-across-tiles paths
+    total = 3 * (1 << 20) + 17
+    nseg = max(2, total // max(1, maxlen // 2))
+    off = make_offsets(nseg, maxlen, total, 21, itype)
+    n = int(off[-1])
+    x = rand(np.int64, n, 22)
+    d, do = dev(stream, x), dev(stream, off)
+    o = A.DeviceArray(nseg, np.int64)
+    for op, seed in ((ADD, None), (MIN, np.int64(5)), (XOR, None)):
+        A.fold_seg(stream, op, seed, d, 1, n, do, nseg, o)
+        assert np.array_equal(o.to_numpy(stream), O.fold_seg(op, seed, x, off.astype(np.int64))), (op, maxlen)
+
+
+def test_fold_seg_float64(stream):
+    off = make_offsets(2000, 5000, 3000000, 31)
+    n = int(off[-1])
+    x = rand(np.float64, n, 32)
+    d, do = dev(stream, x), dev(stream, off)
+    o = A.DeviceArray(2000, np.float64)
+    A.fold_seg(stream, MAX, None, d, 1, n, do, 2000, o)
+    assert np.array_equal(o.to_numpy(stream), O.fold_seg(MAX, None, x, off))
+    A.fold_seg(stream, ADD, None, d, 1, n, do, 2000, o)
+    truth = np.array([math.fsum(x[off[i]:off[i + 1]]) for i in range(2000)])
+    ab = np.add.reduceat(np.abs(x), off[:-1])
+    err = np.abs(o.to_numpy(stream) - truth)
+    assert np.all(err <= 1e-12 * np.maximum(1.0, np.log2(np.maximum(np.diff(off), 2))) * ab + 1e-300)
+
+
+@pytest.mark.parametrize("offset", [1, 3])
+def test_fold_rows_misaligned_view(stream, offset):
+    outer, inner = 7, 10003
+    x = rand(np.int32, outer * inner + offset, 33)
+    d = dev(stream, x)
+    o = A.DeviceArray(outer, np.int32)
+    A.fold(stream, ADD, np.int32(2), d.view(offset, outer * inner), outer, inner, o)
+    want = O.fold(ADD, np.int32(2), x[offset:], outer=outer)
+    assert np.array_equal(o.to_numpy(stream), want)
 
 
 # ------------------------------------------------------------------ permute

@@ -295,3 +295,9 @@ def test_generators():
     u = O.gen_uniform_i32(42, -1000, 1000, 1 << 16)
     assert u.min() == -1000 and u.max() == 1000
     assert np.array_equal(u, O.gen_uniform_i32(42, -1000, 1000, 1 << 16))
+
+
+def test_gen_segment_lengths_dense_stay_positive():
+    for nseg, maxlen, total in ((3145745, 3, 3145745), (1000, 5, 1200), (7, 1000, 7)):
+        lens = O.gen_segment_lengths(21, nseg, maxlen, total)
+        assert lens.min() >= 1 and int(lens.sum()) == total

@@ -19,8 +19,9 @@
 //               starts inside the tile is final at once and written; the one
 //               end that continues a segment from earlier tiles ("open") and
 //               the tile's segmented aggregate go to a per-tile record;
-//   k_seg_fix   one workgroup: segmented scan over the tile aggregates, which
-//               completes the open ends (at most one per tile).
+//   k_seg_fix   a segmented scan over the tile aggregates (256 tiles per
+//               workgroup, chained by a look-back), which completes the open
+//               ends (at most one per tile).
 // A single-pass look-back would chain every tile to its predecessor; with
 // segments shorter than a tile almost every tile needs a carry, and the
 // chain of round trips, not HBM, then bounds the launch.
@@ -35,6 +36,7 @@
 // HBM roofline: n*sizeof(T) + (m+1)*sizeof(I) + m*sizeof(T) bytes.
 #include "dispatch.hpp"
 #include "host.hpp"
+#include "lookback.hpp"
 #include "vecmem.hpp"
 #include "wave64.hpp"
 
@@ -116,7 +118,7 @@ __device__ __forceinline__ int32_t rel_pos(int64_t p, int64_t base, int64_t tile
 }
 
 // One wave per tile: the tile's segment range by two 64-ary searches (one
-// round trip narrows a range 64-fold) and, when they fit, the tile-relative
+// round trip narrows both ranges 64-fold) and, when they fit, the tile-relative
 // positions of its starts.  Then, when a seed is given, every empty segment
 // gets the seed (grid-stride over the segments).
 template <typename T, typename I>
@@ -125,24 +127,31 @@ __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, in
                                                   uint32_t fill_empty, T seed) {
   const uint32_t t = blockIdx.x * 4u + threadIdx.x / 64u;
   const int lane = lane_id();
-  // last g in [0, nseg] with off[g] <= e, 0 if none
-  auto search = [&](int64_t e) -> int64_t {
-    if ((int64_t)off[0] > e) return 0;
-    int64_t lo = 0, hi = nseg;
-    while (hi > lo) {
-      const int64_t step = (hi - lo + 64) / 64;
-      const int64_t g = lo + (int64_t)lane * step;
-      const bool le = g <= hi && (int64_t)off[g] <= e;
-      const int c = __popcll(__ballot(le));  // >= 1: off[lo] <= e
-      lo = lo + (int64_t)(c - 1) * step;
-      hi = min(hi, lo + step - 1);
-    }
-    return lo;
-  };
   if (t < ntiles) {  // whole waves
     const int64_t base = (int64_t)t * tile;  // position
-    const int64_t g_lo = search(base - shift);
-    const int64_t g_hi = search(base + tile - shift);
+    // g_lo, g_hi: last g in [0, nseg] with off[g] <= e for e = the tile's
+    // first and one-past-last row element (0 if none); the two searches
+    // share their round trips
+    const int64_t ea = base - shift, eb = base + tile - shift;
+    const int64_t off0 = (int64_t)off[0];
+    int64_t la = 0, ha = off0 > ea ? 0 : nseg, lb = 0, hb = off0 > eb ? 0 : nseg;
+    while (ha > la || hb > lb) {
+      const int64_t sa = (ha - la + 64) / 64, sb = (hb - lb + 64) / 64;
+      const int64_t ga = la + (int64_t)lane * sa, gb = lb + (int64_t)lane * sb;
+      const bool pa = ha > la && ga <= ha, pb = hb > lb && gb <= hb;
+      const int64_t va = pa ? (int64_t)off[ga] : 0, vb = pb ? (int64_t)off[gb] : 0;
+      if (ha > la) {
+        const int c = __popcll(__ballot(pa && va <= ea));  // >= 1: off[la] <= ea
+        la = la + (int64_t)(c - 1) * sa;
+        ha = min(ha, la + sa - 1);
+      }
+      if (hb > lb) {
+        const int c = __popcll(__ballot(pb && vb <= eb));
+        lb = lb + (int64_t)(c - 1) * sb;
+        hb = min(hb, lb + sb - 1);
+      }
+    }
+    const int64_t g_lo = la, g_hi = lb;
     const int64_t cnt = g_hi - g_lo + 2;
     if (lane == 0) segs[t] = TileSegs{g_lo, cnt};
     if (cnt <= SLOT && lane < cnt) {
@@ -358,45 +367,70 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   }
 }
 
-// One workgroup: segmented exclusive scan over the tile aggregates gives each
-// tile the carry of the segment it starts in; that completes the open ends.
+// Segmented exclusive scan over the tile aggregates: it gives each tile the
+// carry of the segment it starts in, which completes the open ends.  Each
+// workgroup (numbered by ticket) takes BLOCK consecutive tiles, one per
+// thread (coalesced), scans them, publishes its aggregate and takes the
+// carry from the workgroups before it with a decoupled look-back (with at
+// most 64 workgroups one window covers them all).
 template <typename T, typename Op, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_seg_fix(const TileCarry<T>* carry, uint32_t ntiles, T* out,
-                                                   int64_t nseg, uint32_t has_seed, T seed) {
+                                                   int64_t nseg, uint32_t has_seed, T seed, SyncCtl* ctl,
+                                                   uint64_t* status) {
+  constexpr int W = BLOCK / 64;
   using S = SegV<T>;
-  __shared__ S s_part[BLOCK / 64];
+  constexpr int NW = Tagged<S>::NW;
+  __shared__ uint64_t s_tick;
+  __shared__ S s_part[W];
+  __shared__ S s_carry;
   SegOp<Op> sop{Op{}};
   Op op{};
   const S sid{identity<T>(op), 0u};
   const int lane = lane_id(), w = threadIdx.x / 64;
-  const uint32_t per = (ntiles + BLOCK - 1) / BLOCK;
-  const uint32_t t0 = min(threadIdx.x * per, ntiles), t1 = min(t0 + per, ntiles);
-  S own = sid;
-  for (uint32_t t = t0; t < t1; ++t) own = sop(own, carry[t].agg);
-  // exclusive segmented scan of the per-thread combinations
-  const S inc = wave_inclusive_scan(own, sop);
+  if (threadIdx.x == 0) s_tick = take_ticket(ctl);
+  __syncthreads();
+  const uint32_t epoch = (uint32_t)(s_tick >> 32);
+  const uint32_t b = (uint32_t)s_tick;  // < gridDim.x: one ticket per workgroup
+  const uint32_t t = b * BLOCK + threadIdx.x;
+  TileCarry<T> c;
+  if (t < ntiles) c = carry[t];
+  else c = TileCarry<T>{sid, sid.v, -1};
+  const S inc = wave_inclusive_scan(c.agg, sop);
   const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
-  S X = lane > 0 ? up : sid;
   if (lane == 63) s_part[w] = inc;
   __syncthreads();
-  S wp = sid;
+  S wp = sid, agg = sid;
 #pragma unroll
-  for (int i = 0; i < BLOCK / 64; ++i)
-    if (i < w) wp = sop(wp, s_part[i]);
-  X = sop(wp, X);
-  for (uint32_t t = t0; t < t1; ++t) {
-    const TileCarry<T> c = carry[t];
-    if (c.open_g >= 0 && c.open_g < nseg) {
-      T r = op(X.v, c.open_v);
-      if (has_seed) r = op(seed, r);
-      out[c.open_g] = r;
-    }
-    X = sop(X, c.agg);
+  for (int i = 0; i < W; ++i) {
+    const S p = s_part[i];
+    if (i < w) wp = sop(wp, p);
+    agg = sop(agg, p);
   }
+  if (w == 0) {
+    S X = sid;
+    uint64_t* slot = status + (size_t)b * NW;
+    const bool last = b + 1 == gridDim.x;
+    if (b > 0) {
+      if (lane == 0) Tagged<S>::publish(slot, agg, make_tag(epoch, agg.f ? ST_PREFIX : ST_AGGREGATE));
+      X = wave_lookback<S>(status, b, epoch, sop);
+      if (lane == 0 && !agg.f && !last) Tagged<S>::publish(slot, sop(X, agg), make_tag(epoch, ST_PREFIX));
+    } else if (lane == 0 && !last) {
+      Tagged<S>::publish(slot, agg, make_tag(epoch, ST_PREFIX));
+    }
+    if (lane == 0) s_carry = X;
+  }
+  __syncthreads();
+  const S X = sop(sop(s_carry, wp), lane > 0 ? up : sid);  // carry into this thread's tile
+  if (c.open_g >= 0 && c.open_g < nseg) {
+    T v = op(X.v, c.open_v);
+    if (has_seed) v = op(seed, v);
+    out[c.open_g] = v;
+  }
+  block_retire(ctl, gridDim.x, epoch);
 }
 
 #ifndef ACC_SEG_K
-#define ACC_SEG_K 16
+#define ACC_SEG_K 8
 #endif
 template <typename T> struct SegTune {
   static constexpr int BLOCK = 256;
@@ -427,7 +461,11 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
   const size_t carry_bytes = ((size_t)ntiles * sizeof(TileCarry<T>) + 255) / 256 * 256;
   const size_t segs_bytes = offsets ? ((size_t)ntiles * sizeof(TileSegs) + 255) / 256 * 256 : 0;
   const size_t slot_bytes = offsets ? (size_t)ntiles * SLOT * sizeof(int32_t) : 0;
-  ACC_CHECK(stream_workspace(s, carry_bytes + segs_bytes + slot_bytes, &ctl, &ws));
+  constexpr int FIX = 256;  // tiles per k_seg_fix workgroup
+  const int64_t nfix = ceil_div(ntiles, (int64_t)FIX);
+  const size_t fix_bytes = (size_t)nfix * Tagged<SegV<T>>::NW * 8;
+  ACC_CHECK(stream_workspace(s, carry_bytes + segs_bytes + slot_bytes + fix_bytes, &ctl, &ws));
+  uint64_t* fix_status = (uint64_t*)((char*)ws + carry_bytes + segs_bytes + slot_bytes);
   TileCarry<T>* carry = (TileCarry<T>*)ws;
   const bool fill_empty = seed && check_empty;
   if constexpr (!std::is_same<Starts, UniformStarts>::value) {
@@ -451,8 +489,9 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
                      a);
   ACC_HIP_CHECK(hipGetLastError());
   if (ntiles > 1) {
-    hipLaunchKernelGGL((k_seg_fix<T, Op, 1024>), dim3(1), dim3(1024), 0, s->stream, (const TileCarry<T>*)carry,
-                       (uint32_t)ntiles, out, nseg, seed ? 1u : 0u, seed ? *seed : T());
+    hipLaunchKernelGGL((k_seg_fix<T, Op, FIX>), dim3((unsigned)nfix), dim3(FIX), 0, s->stream,
+                       (const TileCarry<T>*)carry, (uint32_t)ntiles, out, nseg, seed ? 1u : 0u, seed ? *seed : T(),
+                       ctl, fix_status);
     ACC_HIP_CHECK(hipGetLastError());
   }
   return ACC_SUCCESS;

@@ -166,15 +166,33 @@ __global__ __launch_bounds__(BLOCK) void k_permute_lds(PermArgs<T, I> a) {
     }
   };
 
+  // U index vectors (and their source values) in flight per thread before
+  // the LDS updates: one load at a time leaves the loop latency-bound
+  constexpr int U = CONST_SRC ? 8 : 4;
   const I* body = a.idx + a.head;
   const int64_t vb = (int64_t)blockIdx.x * a.chunk;
   const int64_t ve = min(vb + a.chunk, a.nvec);
-  for (int64_t i = vb + threadIdx.x; i < ve; i += BLOCK) {
-    Vec<I> iv = load16_stream(body + i * VI);
+  for (int64_t i0 = vb + threadIdx.x; i0 < ve; i0 += (int64_t)BLOCK * U) {
+    Vec<I> iv[U];
+    T sv[U][VI];
 #pragma unroll
-    for (int k = 0; k < VI; ++k) {
-      const int64_t e = a.head + i * VI + k;
-      update(iv.e[k], CONST_SRC ? a.fill : a.src[e]);
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * BLOCK;
+      if (i < ve) {
+        iv[u] = load16_stream(body + i * VI);
+        if (!CONST_SRC) {
+#pragma unroll
+          for (int k = 0; k < VI; ++k) sv[u][k] = a.src[a.head + i * VI + k];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * BLOCK;
+      if (i < ve) {
+#pragma unroll
+        for (int k = 0; k < VI; ++k) update(iv[u].e[k], CONST_SRC ? a.fill : sv[u][k]);
+      }
     }
   }
   // head and tail elements (outside whole vectors): the last workgroup

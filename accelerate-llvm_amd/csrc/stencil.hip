@@ -4,11 +4,11 @@
 // border kernel per face (2*rank launches on forked streams joined by
 // events, PTX/Execute.hs:686-776), and every tap is a separate global load
 // (PTX/CodeGen/Stencil.hs:113-177).  Here one launch covers interior and
-// border: each workgroup stages a (TH+2RY) x (TW+2RX) halo tile in LDS,
-// applying the boundary map (LLVM/CodeGen/Stencil.hs:224-253) to the
-// coordinates outside the array as it loads, then each thread computes a
-// 4x4 block of outputs from a register window over the LDS tile and writes
-// it with dwordx4 stores.
+// border: each workgroup stages a (TH+2RY) x (TW+2RX) halo tile in LDS
+// (interior tiles with dwordx4 copies, border tiles applying the boundary map,
+// LLVM/CodeGen/Stencil.hs:224-253, to the coordinates outside the array as
+// they load), then each thread computes a 4x4 block of outputs from a register
+// window read from LDS with 16-byte reads and writes it with dwordx4 stores.
 //
 // The stencil function is f(tuple) = sum_{dy,dx} w[dy][dx] * x[y+dy][x+dx]
 // with taps in tuple order (rows outer, columns inner,
@@ -33,6 +33,9 @@ template <typename T> struct StArgs {
   T* out;
   int64_t h, w;
   uint32_t tiles_x, tiles_y;
+  // interior tiles: tx in [ix0, ix1), ty in [iy0, iy1) (empty when the
+  // rows are not 16-byte aligned); the border launch covers the others
+  uint32_t ix0, ix1, iy0, iy1;
   int boundary;
   T cval;
   T wt[81];  // (2RY+1) x (2RX+1) weights, row-major
@@ -52,12 +55,24 @@ __device__ __forceinline__ int64_t bound_index(int b, int64_t i, int64_t n) {
   return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
-template <typename T, int RY, int RX>
+// Interior tiles (INTERIOR): the halo, rounded out to 16-byte boundaries,
+// lies inside the array and is copied with dwordx4 vectors.  Border tiles:
+// coordinates outside the array go through the boundary map (constant: the
+// value itself), element by element.  The two are separate launches: one
+// kernel branching between both staging loops has been compiled for gfx950
+// with the register holding the thread index left undefined on one path.
+template <typename T, int RY, int RX, bool INTERIOR>
 __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
-  constexpr int LW = ST_TW + 2 * RX;  // LDS row length
+  constexpr int VEC = Vec<T>::N;
+  constexpr int PADL = (RX + VEC - 1) / VEC * VEC;  // halo rounded out to 16-byte boundaries
+  constexpr int CO = PADL - RX;                     // halo column c sits at LDS column c + CO
+  constexpr int LS = ST_TW + 2 * PADL;              // LDS row stride: a 16-byte multiple
+  constexpr int LW = ST_TW + 2 * RX;                // halo row length
   constexpr int LH = ST_TH + 2 * RY;
   constexpr int KW = 2 * RX + 1;
-  __shared__ __align__(16) T tile[LH * LW];
+  constexpr int WIN = ST_PX + 2 * RX;               // register window row
+  constexpr int NVW = (CO + WIN + VEC - 1) / VEC;   // LDS vectors covering it
+  __shared__ __align__(16) T tile[LH * LS];
 
   // XCD-aware tile order: consecutive workgroups land on different XCDs, so
   // when the tile count allows, each XCD gets a contiguous band of tiles,
@@ -65,26 +80,57 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   // and share, in its L2, the 128-byte lines that hold each row's horizontal
   // halo (a column-major walk fetched every such line twice from HBM); the
   // vertical halo is 2*RY of 64 rows.
-  const uint32_t nt = a.tiles_x * a.tiles_y;
-  uint32_t b = blockIdx.x;
-  if (nt % 8u == 0u) b = (b % 8u) * (nt / 8u) + b / 8u;
-  const uint32_t ty = b / a.tiles_x;  // row-major
-  const uint32_t tx = b - ty * a.tiles_x;
+  uint32_t tx, ty;
+  if constexpr (INTERIOR) {
+    const uint32_t nx = a.ix1 - a.ix0, nt = nx * (a.iy1 - a.iy0);
+    uint32_t b = blockIdx.x;
+    if (nt % 8u == 0u) b = (b % 8u) * (nt / 8u) + b / 8u;
+    ty = a.iy0 + b / nx;
+    tx = a.ix0 + b % nx;
+  } else {
+    // the frame around the interior rectangle: rows above it, rows below
+    // it, then the left and right parts of the rows beside it
+    uint32_t b = blockIdx.x;
+    const uint32_t top = a.iy0 * a.tiles_x, bottom = (a.tiles_y - a.iy1) * a.tiles_x;
+    if (b < top) {
+      ty = b / a.tiles_x;
+      tx = b % a.tiles_x;
+    } else if (b - top < bottom) {
+      b -= top;
+      ty = a.iy1 + b / a.tiles_x;
+      tx = b % a.tiles_x;
+    } else {
+      b -= top + bottom;
+      const uint32_t left = a.ix0, side = a.ix0 + (a.tiles_x - a.ix1);
+      ty = a.iy0 + b / side;
+      const uint32_t k = b % side;
+      tx = k < left ? k : a.ix1 + (k - left);
+    }
+  }
   const int64_t x0 = (int64_t)tx * ST_TW, y0 = (int64_t)ty * ST_TH;
 
-  // ---- stage the halo tile; coordinates outside the array go through the
-  // boundary map (constant: the value itself)
-  const bool constant = a.boundary == ACC_BOUNDARY_CONSTANT;
-  for (int i = threadIdx.x; i < LH * LW; i += ST_BLOCK) {
-    const int r = i / LW, c = i - r * LW;
-    int64_t gy = y0 - RY + r, gx = x0 - RX + c;
-    const bool inside = (uint64_t)gy < (uint64_t)a.h && (uint64_t)gx < (uint64_t)a.w;
-    if (!inside) {
-      gy = bound_index(a.boundary, gy, a.h);
-      gx = bound_index(a.boundary, gx, a.w);
+  // ---- stage the halo tile
+  if constexpr (INTERIOR) {
+    constexpr int NV = LS / VEC;  // vectors per LDS row
+    const T* src = a.in + (y0 - RY) * a.w + (x0 - PADL);
+    for (int i = threadIdx.x; i < LH * NV; i += ST_BLOCK) {
+      const int r = i / NV, q = i - r * NV;
+      const Vec<T> v = load16(src + (int64_t)r * a.w + q * VEC);
+      *(Vec<T>*)&tile[r * LS + q * VEC] = v;
     }
-    T v = a.in[gy * a.w + gx];
-    tile[i] = (inside || !constant) ? v : a.cval;
+  } else {
+    const bool constant = a.boundary == ACC_BOUNDARY_CONSTANT;
+    for (int i = threadIdx.x; i < LH * LW; i += ST_BLOCK) {
+      const int r = i / LW, c = i - r * LW;
+      int64_t gy = y0 - RY + r, gx = x0 - RX + c;
+      const bool inside = (uint64_t)gy < (uint64_t)a.h && (uint64_t)gx < (uint64_t)a.w;
+      if (!inside) {
+        gy = bound_index(a.boundary, gy, a.h);
+        gx = bound_index(a.boundary, gx, a.w);
+      }
+      T v = a.in[gy * a.w + gx];
+      tile[r * LS + c + CO] = (inside || !constant) ? v : a.cval;
+    }
   }
   __syncthreads();
 
@@ -94,10 +140,17 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   T acc[ST_PY][ST_PX];
 #pragma unroll
   for (int lr = 0; lr < ST_PY + 2 * RY; ++lr) {
-    T v[ST_PX + 2 * RX];
-    const T* row = tile + (cy + lr) * LW + cx;
+    // the window row from 16-byte LDS reads (cx is a multiple of VEC)
+    T wv[NVW * VEC];
 #pragma unroll
-    for (int c = 0; c < ST_PX + 2 * RX; ++c) v[c] = row[c];
+    for (int q = 0; q < NVW; ++q) {
+      const Vec<T> r = *(const Vec<T>*)&tile[(cy + lr) * LS + cx + q * VEC];
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) wv[q * VEC + j] = r.e[j];
+    }
+    T v[WIN];
+#pragma unroll
+    for (int c = 0; c < WIN; ++c) v[c] = wv[CO + c];
 #pragma unroll
     for (int oy = 0; oy < ST_PY; ++oy) {
       const int dy = lr - oy;  // tap row 0..2RY
@@ -140,6 +193,8 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
 template <typename T, int RY, int RX>
 int stencil_launch(acc_stream_t s, const T* w, int boundary, const T* cval, const T* in, int64_t h,
                    int64_t wd, T* out) {
+  constexpr int VEC = Vec<T>::N;
+  constexpr int64_t PADL = (RX + VEC - 1) / VEC * VEC;
   StArgs<T> a;
   a.in = in;
   a.out = out;
@@ -149,12 +204,25 @@ int stencil_launch(acc_stream_t s, const T* w, int boundary, const T* cval, cons
   if (tiles_x * tiles_y > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "stencil: array too large");
   a.tiles_x = (uint32_t)tiles_x;
   a.tiles_y = (uint32_t)tiles_y;
+  // interior tiles: x0 >= PADL, x0 + TW + PADL <= w, y0 >= RY, y0 + TH + RY <= h,
+  // with 16-byte aligned rows
+  int64_t ix0 = ceil_div(PADL, (int64_t)ST_TW), ix1 = wd >= ST_TW + PADL ? (wd - ST_TW - PADL) / ST_TW + 1 : 0;
+  int64_t iy0 = ceil_div((int64_t)RY, (int64_t)ST_TH), iy1 = h >= ST_TH + RY ? (h - ST_TH - RY) / ST_TH + 1 : 0;
+  const bool aligned = (uintptr_t)in % 16 == 0 && (wd * (int64_t)sizeof(T)) % 16 == 0;
+  if (!aligned || ix1 <= ix0 || iy1 <= iy0) ix0 = ix1 = iy0 = iy1 = 0;
+  a.ix0 = (uint32_t)ix0;
+  a.ix1 = (uint32_t)ix1;
+  a.iy0 = (uint32_t)iy0;
+  a.iy1 = (uint32_t)iy1;
   a.boundary = boundary;
   a.cval = cval ? *cval : T();
   const int taps = (2 * RY + 1) * (2 * RX + 1);
   for (int i = 0; i < 81; ++i) a.wt[i] = i < taps ? w[i] : T();
   const int64_t nt = tiles_x * tiles_y;
-  hipLaunchKernelGGL((k_stencil2d<T, RY, RX>), dim3((unsigned)nt), dim3(ST_BLOCK), 0, s->stream, a);
+  const int64_t ni = (ix1 - ix0) * (iy1 - iy0);
+  if (ni > 0) hipLaunchKernelGGL((k_stencil2d<T, RY, RX, true>), dim3((unsigned)ni), dim3(ST_BLOCK), 0, s->stream, a);
+  if (nt > ni)
+    hipLaunchKernelGGL((k_stencil2d<T, RY, RX, false>), dim3((unsigned)(nt - ni)), dim3(ST_BLOCK), 0, s->stream, a);
   ACC_HIP_CHECK(hipGetLastError());
   return ACC_SUCCESS;
 }

@@ -471,6 +471,22 @@ def test_stencil_int_exact(stream, b, dt):
     assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(1, 1, wt, b, dt(5), img))
 
 
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+@pytest.mark.parametrize("r", [(1, 1), (2, 3), (4, 4), (3, 1)])
+@pytest.mark.parametrize("shape", [(300, 512), (200, 516), (131, 515)])
+def test_stencil_int_interior_tiles(stream, dt, r, shape):
+    # widths with 16-byte rows take the vector-staged interior tiles (the
+    # last shape's rows are not 16-byte aligned: element staging everywhere)
+    ry, rx = r
+    h, w = shape
+    img = np.random.default_rng(h + w).integers(-1000, 1000, size=(h, w)).astype(dt)
+    wt = np.random.default_rng(7).integers(-3, 4, size=(2 * ry + 1, 2 * rx + 1)).astype(dt)
+    out = A.DeviceArray(h * w, dt)
+    for b in (CLAMP, WRAP):
+        A.stencil2d(stream, ry, rx, wt, b, None, dev(stream, img), h, w, out)
+        assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(ry, rx, wt, b, None, img)), b
+
+
 def test_stencil_float64_special(stream):
     h, w = 70, 90
     img = np.full((h, w), 3.0)

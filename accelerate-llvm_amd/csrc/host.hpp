@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <string>
 #include <mutex>
+#include <vector>
 #include "../../include/accelerate_hip.h"
 
 namespace acc {
@@ -52,6 +53,9 @@ struct acc_stream_s {
   void* ws = nullptr;          // device: [SyncCtl (256 B) | status words]
   size_t ws_status_bytes = 0;  // capacity of the status region
   bool capturing = false;
+  // workspaces outgrown by this stream: freed when the stream is destroyed
+  // (hipFree would synchronise the whole device at the moment of growth)
+  std::vector<void*> retired;
 };
 
 struct acc_event_s {

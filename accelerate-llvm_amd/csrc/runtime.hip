@@ -151,11 +151,9 @@ int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_
     void* ws = nullptr;
     ACC_CHECK(dev_malloc(s->device, kCtlBytes + cap, &ws));
     ACC_HIP_CHECK(hipMemsetAsync(ws, 0, kCtlBytes + cap, s->stream));
-    if (s->ws) {
-      // the old workspace may still be in use by queued work
-      ACC_HIP_CHECK(hipStreamSynchronize(s->stream));
-      ACC_CHECK(dev_free(s->device, s->ws));
-    }
+    // the old workspace may still be in use by queued work: it is retired,
+    // not freed (no synchronisation of the stream or the device here)
+    if (s->ws) s->retired.push_back(s->ws);
     s->ws = ws;
     s->ws_status_bytes = cap;
   }
@@ -321,6 +319,7 @@ int acc_hip_stream_destroy(acc_stream_t s) {
   if (rc == ACC_SUCCESS) {
     (void)hipStreamSynchronize(s->stream);
     if (s->ws) dev_free(s->device, s->ws);
+    for (void* p : s->retired) dev_free(s->device, p);
     (void)hipStreamDestroy(s->stream);
   }
   delete s;

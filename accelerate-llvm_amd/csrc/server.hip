@@ -273,6 +273,9 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
     int rc = acc_hip_stream_create(device, &st);
     if (rc) return rc;
     s->slots.push_back(st);
+    // skeleton workspace up front (enough for foldSeg over ~2^29 elements),
+    // so that no request grows it while others are in flight
+    ACC_CHECK(acc_hip_stream_reserve(st, (size_t)16 << 20));
     hipEvent_t e;
     ACC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s->slot_event.push_back(e);

@@ -82,9 +82,11 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   // vertical halo is 2*RY of 64 rows.
   uint32_t tx, ty;
   if constexpr (INTERIOR) {
+    // workgroup b runs on XCD b % 8; XCD k takes tiles [start(k), start(k+1))
+    // with nt / 8 tiles each and one more for the first nt % 8 XCDs
     const uint32_t nx = a.ix1 - a.ix0, nt = nx * (a.iy1 - a.iy0);
-    uint32_t b = blockIdx.x;
-    if (nt % 8u == 0u) b = (b % 8u) * (nt / 8u) + b / 8u;
+    const uint32_t q = nt / 8u, r = nt % 8u, xcd = blockIdx.x % 8u;
+    const uint32_t b = xcd * q + min(xcd, r) + blockIdx.x / 8u;
     ty = a.iy0 + b / nx;
     tx = a.ix0 + b % nx;
   } else {

@@ -281,10 +281,16 @@ def kernel_only():
     return 0
 
 
+# the kernels of one fold1Seg call: per-tile segment ranges, the tiles, the
+# carries between tiles
+FOLD_SEG_KERNELS = ("k_tile_seg", "k_fold_seg", "k_seg_fix")
+
+
 def measure_traffic():
-    """HBM bytes per fold1Seg (+) launch from two rocprofv3 counter runs
-    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled so that the 1 GiB
-    device copy in the same run reads/writes exactly 1 GiB.  None on failure."""
+    """HBM bytes per fold1Seg (+) call (its three kernels) from two rocprofv3
+    counter runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled so
+    that the 1 GiB device copy in the same run reads/writes exactly 1 GiB.
+    None on failure."""
     import csv
     import shutil
     import signal
@@ -310,20 +316,22 @@ def measure_traffic():
         files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if not files:
             return None
-        seg, cp = [], []
+        seg, cp = {k: [] for k in FOLD_SEG_KERNELS}, []
         for row in csv.DictReader(open(files[0])):
             if row["Counter_Name"] != counter:
                 continue
-            if row["Kernel_Name"].startswith("k_fold_seg"):
-                seg.append(float(row["Counter_Value"]))
+            name = row["Kernel_Name"].split("<")[0].split("(")[0].split("::")[-1]
+            if name in seg:
+                seg[name].append(float(row["Counter_Value"]))
             elif "copyBuffer" in row["Kernel_Name"] and int(row["Grid_Size"]) >= 65536:
                 cp.append(float(row["Counter_Value"]))
         shutil.rmtree(d, ignore_errors=True)
-        if not seg or not cp:
+        if not all(seg.values()) or not cp:
             return None
         scale = (1 << 30) / (sum(cp) / len(cp) * 1024.0)
-        vals[counter] = sum(seg) / len(seg) * 1024.0 * scale
-        log(f"{counter}: {sum(seg) / len(seg):.0f} KiB per fold1Seg launch, scale {scale:.3f}")
+        kib = sum(sum(v) / len(v) for v in seg.values())  # one fold1Seg call: its three kernels
+        vals[counter] = kib * 1024.0 * scale
+        log(f"{counter}: {kib:.0f} KiB per fold1Seg call, scale {scale:.3f}")
     return int(vals["FETCH_SIZE"] + vals["WRITE_SIZE"])
 
 
@@ -415,7 +423,7 @@ def main():
     k_ms = timed(s, lambda: A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum), 50)
     k_gbs = k_bytes / (k_ms * 1e-3) / 1e9
     roofline = {
-        "kernel": "k_fold_seg<float, OpAdd, OffsetStarts<long>> (timed with its k_tile_seg pre-pass)",
+        "kernel": "fold1Seg (+) Float32 call: k_tile_seg + k_fold_seg + k_seg_fix",
         "bound": "hbm",
         "achieved": round(k_gbs, 1),
         "peak": HBM_PEAK_GBS,

@@ -1,8 +1,8 @@
 // Serving loop: requests (one skeleton invocation each, inputs resident on
-// the device) arrive over time; a scheduler thread admits them as execution
-// slots and device memory free up, runs admitted requests concurrently on
-// per-slot streams, and records each request's latency from arrival to its
-// result being complete on the device.
+// the device) arrive over time; one worker thread per execution slot admits
+// them as slots and device memory free up, runs admitted requests
+// concurrently on per-slot streams, and records each request's latency from
+// arrival to its result being complete on the device.
 //
 // This is continuous batching in the sense of the serving literature: there
 // is no fixed batch boundary; a request that arrives while others execute is
@@ -119,7 +119,7 @@ struct acc_server_s {
   uint64_t completed = 0, failed = 0;
   double bytes_done = 0;
   bool stop = false;
-  std::thread worker;
+  std::vector<std::thread> workers;  // one per slot
 };
 
 namespace acc {
@@ -176,78 +176,75 @@ static int run_request(acc_server_t s, Req& q, acc_stream_t st, void* out) {
   return fail(ACC_ERROR_UNSUPPORTED, "server: request kind %d", r.skeleton);
 }
 
-static void server_loop(acc_server_t s) {
+// One worker thread per slot: it takes the earliest arrived request whose
+// output fits in the free arena (a request that does not fit is passed by
+// later ones), launches it on the slot's stream, waits for its completion
+// event and retires it.  Workers launch in parallel, so the host-side cost of
+// the launches (a few microseconds each) is spread over the slots.
+static void slot_worker(acc_server_t s, int slot) {
   (void)hipSetDevice(s->device);
+  acc_stream_t st = s->slots[slot];
+  hipEvent_t ev = s->slot_event[slot];
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     const auto now = clk::now();
-    // completions
-    for (size_t k = 0; k < s->slots.size(); ++k) {
-      const uint64_t id = s->slot_req[k];
-      if (!id) continue;
-      if (hipEventQuery(s->slot_event[k]) != hipSuccess) continue;
-      Req& q = *s->reqs[id];
+    Req* pick = nullptr;
+    size_t off = 0;
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+      Req& q = *s->reqs[*it];
+      if (q.arrival > now) break;  // pending is ordered by arrival
+      if (!s->arena.alloc(q.out_bytes, &off)) continue;
+      s->pending.erase(it);
+      pick = &q;
+      break;
+    }
+    if (pick) {
+      Req& q = *pick;
+      q.out_off = off;
+      q.slot = slot;
+      s->slot_req[slot] = q.id;
+      void* out = (char*)s->arena_base + off;
+      lk.unlock();
+      int rc = run_request(s, q, st, out);
+      if (!rc && hipEventRecord(ev, st->stream) != hipSuccess) rc = ACC_ERROR_HIP;
+      while (!rc) {  // poll: a blocking wait adds the scheduler's wake-up latency
+        const hipError_t e = hipEventQuery(ev);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) rc = ACC_ERROR_HIP;
+        else std::this_thread::yield();
+      }
+      lk.lock();
       q.done = clk::now();
       q.finished = true;
-      s->slot_req[k] = 0;
-      s->completed++;
-      s->latencies_us.push_back(std::chrono::duration<double, std::micro>(q.done - q.arrival).count());
-      s->bytes_done += request_bytes(q.r);
-      if (!s->any || q.done > s->last_done) s->last_done = q.done;
-      if (!q.keep) {
-        s->arena.release(q.out_off, q.out_bytes);
-        s->reqs.erase(id);
+      s->slot_req[slot] = 0;
+      if (rc) {
+        q.status = rc;
+        s->failed++;
+        s->arena.release(off, q.out_bytes);
+      } else {
+        s->completed++;
+        s->latencies_us.push_back(std::chrono::duration<double, std::micro>(q.done - q.arrival).count());
+        s->bytes_done += request_bytes(q.r);
+        if (!s->any || q.done > s->last_done) s->last_done = q.done;
+        if (!q.keep) {
+          s->arena.release(q.out_off, q.out_bytes);
+          s->reqs.erase(q.id);  // q is gone from here on
+        }
       }
       s->cv.notify_all();
+      continue;
     }
-    // admissions: requests that have arrived, in arrival order; one that does
-    // not fit in the free memory is passed by later ones that do.  The scan
-    // restarts after each admission (the lock is dropped while launching).
-    bool admitted = true;
-    while (admitted) {
-      admitted = false;
-      int slot = -1;
-      for (size_t k = 0; k < s->slots.size(); ++k)
-        if (!s->slot_req[k]) { slot = (int)k; break; }
-      if (slot < 0) break;
-      for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
-        Req& q = *s->reqs[*it];
-        if (q.arrival > now) break;  // pending is ordered by arrival
-        size_t off;
-        if (!s->arena.alloc(q.out_bytes, &off)) continue;
-        s->pending.erase(it);
-        q.out_off = off;
-        q.slot = slot;
-        s->slot_req[slot] = q.id;  // reserve the slot while launching
-        void* out = (char*)s->arena_base + off;
-        acc_stream_t st = s->slots[slot];
-        lk.unlock();
-        int rc = run_request(s, q, st, out);
-        if (!rc && hipEventRecord(s->slot_event[slot], st->stream) != hipSuccess) rc = ACC_ERROR_HIP;
-        lk.lock();
-        if (rc) {
-          s->slot_req[slot] = 0;
-          q.status = rc;
-          q.finished = true;
-          q.done = clk::now();
-          s->failed++;
-          s->arena.release(off, q.out_bytes);
-          s->cv.notify_all();
-        }
-        admitted = true;
-        break;
-      }
-    }
-    bool busy = false;
-    for (auto id : s->slot_req) busy |= id != 0;
-    if (s->stop && !busy && s->pending.empty()) break;
-    if (!busy && s->pending.empty()) {
+    if (s->stop && s->pending.empty()) break;
+    if (s->pending.empty()) {
       s->cv.wait_for(lk, std::chrono::milliseconds(5));
-    } else if (!busy && !s->pending.empty() && s->reqs[s->pending.front()]->arrival > clk::now()) {
-      s->cv.wait_until(lk, s->reqs[s->pending.front()]->arrival);
+    } else if (s->reqs[s->pending.front()]->arrival - clk::now() > std::chrono::microseconds(300)) {
+      // idle until shortly before the next arrival (timed waits overshoot by
+      // tens of microseconds), then poll
+      s->cv.wait_until(lk, s->reqs[s->pending.front()]->arrival - std::chrono::microseconds(200));
     } else {
+      // the next request is due, or does not fit yet: poll again at once
       lk.unlock();
-      std::this_thread::sleep_for(std::chrono::microseconds(2));
+      std::this_thread::yield();
       lk.lock();
     }
   }
@@ -282,7 +279,7 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
     s->slot_req.push_back(0);
   }
   acc_server_s* raw = s.release();
-  raw->worker = std::thread(server_loop, raw);
+  for (int k = 0; k < slots; ++k) raw->workers.emplace_back(slot_worker, raw, k);
   *out = raw;
   return ACC_SUCCESS;
 }
@@ -402,7 +399,8 @@ int acc_hip_server_destroy(acc_server_t s) {
     s->stop = true;
     s->cv.notify_all();
   }
-  if (s->worker.joinable()) s->worker.join();
+  for (auto& w : s->workers)
+    if (w.joinable()) w.join();
   for (auto st : s->slots) acc_hip_stream_destroy(st);
   for (auto e : s->slot_event) (void)hipEventDestroy(e);
   if (s->arena_base) dev_free(s->device, s->arena_base);

@@ -208,7 +208,7 @@ def bench_serving(s, args):
         s.synchronize()
     plain_s = time.perf_counter() - plain_t0
 
-    srv = A.Server(0, slots=8)
+    srv = A.Server(0, slots=args.serve_slots)
     # capacity: every request arrives at once
     for q in reqs:
         srv.submit(q)
@@ -216,17 +216,19 @@ def bench_serving(s, args):
     cap = srv.stats()
     capacity = cap.req_per_s
     srv.reset_stats()
+    # open-loop Poisson trace at 80% of capacity: arrival times are measured
+    # from one origin (each submit gives its offset from the moment it is made)
     lam = 0.8 * capacity
-    t = 0.0
-    for q in reqs:
-        t += rng.exponential(1.0 / lam)
-        srv.submit(q, arrival_offset_ns=int(t * 1e9))
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs))) + 2e-3
+    t0 = time.perf_counter()
+    for q, at in zip(reqs, arrivals):
+        srv.submit(q, arrival_offset_ns=max(0, int((at - (time.perf_counter() - t0)) * 1e9)))
     srv.drain()
     st = srv.stats()
     srv.destroy()
     out = {
         "requests": R,
-        "slots": 8,
+        "slots": args.serve_slots,
         "capacity_req_per_s": round(capacity, 1),
         "arrival_rate_req_per_s": round(lam, 1),
         "req_per_s": round(st.req_per_s, 1),
@@ -348,6 +350,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batch-problems", type=int, default=10000)
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-slots", type=int, default=8, help="server execution slots (one worker thread each)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--kernel-only", action="store_true",
                     help="(internal) run only the dominant kernel and a 1 GiB copy, for counter collection")

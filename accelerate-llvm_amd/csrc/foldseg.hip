@@ -37,26 +37,11 @@
 #include "dispatch.hpp"
 #include "host.hpp"
 #include "lookback.hpp"
+#include "seg.hpp"
 #include "vecmem.hpp"
 #include "wave64.hpp"
 
 namespace acc {
-
-template <typename T> struct SegV {
-  T v;
-  uint32_t f;  // a segment starts at (or after) the first element
-};
-
-template <typename Op> struct SegOp {
-  Op op;
-  template <typename T>
-  __device__ __forceinline__ SegV<T> operator()(SegV<T> a, SegV<T> b) const {
-    SegV<T> r;
-    r.v = b.f ? b.v : op(a.v, b.v);
-    r.f = a.f | b.f;
-    return r;
-  }
-};
 
 // Element positions inside the kernels are relative to the 16-byte boundary
 // at or below the row's first element: position p is row element p - shift,

@@ -92,3 +92,8 @@ size_t type_size(acc_type t);
 int blocks_per_cu(const void* kernel, int block);
 
 }  // namespace acc
+
+// All rows of a rank >= 2 scan in one launch (scan_rows.hip).
+int acc_scan_rows(acc_stream_t s, acc_type t, acc_op op, bool reverse, bool exclusive, const void* seed,
+                  const void* in, int64_t outer, int64_t inner, void* out, int64_t out_row, void* sums,
+                  bool seed_at_end);

@@ -144,10 +144,14 @@ int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const 
   return aligned ? scan_launch<T, Op, false, true>(s, a) : scan_launch<T, Op, false, false>(s, a);
 }
 
-// rank >= 2 scans: one call per row for now (each row is an independent scan)
+// rank >= 2 scans: rows shorter than 2^20 elements in one fused launch
+// (scan_rows.hip); longer rows one plain scan each, where the launch is
+// amortised and the unsegmented tile is faster
 template <typename T, typename Op>
-int scan_rows(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const T* in,
+int scan_rows(acc_stream_t s, acc_type ty, acc_op opc, bool reverse, bool exclusive, const T* seed, const T* in,
               int64_t outer, int64_t inner, T* out, int64_t out_row, T* sums, bool seed_at_end) {
+  if (outer > 1 && inner < ((int64_t)1 << 20))
+    return acc_scan_rows(s, ty, opc, reverse, exclusive, seed, in, outer, inner, out, out_row, sums, seed_at_end);
   for (int64_t r = 0; r < outer; ++r) {
     T* o = out + r * out_row;
     T* total = sums ? sums + r : (exclusive && !reverse && !sums ? o + inner : nullptr);
@@ -181,9 +185,9 @@ extern "C" int acc_hip_scan(acc_stream_t s, acc_type t, acc_op op, acc_direction
     return with_op<T>(op, [&](auto f) {
       using Op = decltype(f);
       const T* sd = (const T*)seed;
-      if (!sd) return scan_rows<T, Op>(s, reverse, false, nullptr, (const T*)in, outer, inner, (T*)out, inner, nullptr, false);
+      if (!sd) return scan_rows<T, Op>(s, t, op, reverse, false, nullptr, (const T*)in, outer, inner, (T*)out, inner, nullptr, false);
       // scanl z: exclusive values + total at out[n]; scanr z: inclusive values + z at out[n]
-      return scan_rows<T, Op>(s, reverse, !reverse, sd, (const T*)in, outer, inner, (T*)out, inner + 1,
+      return scan_rows<T, Op>(s, t, op, reverse, !reverse, sd, (const T*)in, outer, inner, (T*)out, inner + 1,
                               nullptr, reverse);
     });
   });
@@ -206,7 +210,7 @@ extern "C" int acc_hip_scan_prime(acc_stream_t s, acc_type t, acc_op op, acc_dir
     using T = decltype(z);
     return with_op<T>(op, [&](auto f) {
       using Op = decltype(f);
-      return scan_rows<T, Op>(s, reverse, true, (const T*)seed, (const T*)in, outer, inner, (T*)out, inner,
+      return scan_rows<T, Op>(s, t, op, reverse, true, (const T*)seed, (const T*)in, outer, inner, (T*)out, inner,
                               (T*)sums, false);
     });
   });

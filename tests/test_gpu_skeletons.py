@@ -211,6 +211,53 @@ def test_scan_rows(stream, shape):
         assert np.array_equal(e, we) and np.array_equal(s, ws)
 
 
+ROW_SHAPES = [(1000, 3), (4096, 17), (300, 1), (9, 1000), (3, 100003), (2, 1 << 20), (70000, 2)]
+
+
+@pytest.mark.parametrize("shape", ROW_SHAPES)
+@pytest.mark.parametrize("direction", [L2R, R2L])
+def test_scan_rows_all_modes(stream, shape, direction):
+    # rank-2 scans along the innermost dimension (one fused launch below 2^20
+    # columns, one scan per row above)
+    outer, inner = shape
+    x = rand(np.int32, outer * inner, outer + inner)
+    for op in (ADD, MAX):
+        assert np.array_equal(gpu_scan(stream, op, direction, None, x, outer),
+                              O.scan(op, direction, None, x, outer)), (op, "incl")
+        z = np.int32(-77)
+        assert np.array_equal(gpu_scan(stream, op, direction, z, x, outer),
+                              O.scan(op, direction, z, x, outer)), (op, "excl")
+        e, sm = gpu_scan_prime(stream, op, direction, z, x, outer)
+        we, ws = O.scan_prime(op, direction, z, x, outer)
+        assert np.array_equal(e, we) and np.array_equal(sm, ws), (op, "scan'")
+
+
+@pytest.mark.parametrize("dt", [np.int64, np.uint32, np.float64])
+@pytest.mark.parametrize("shape", [(513, 31), (5, 40001)])
+def test_scan_rows_other_types(stream, dt, shape):
+    outer, inner = shape
+    x = rand(dt, outer * inner, 11)
+    for direction in (L2R, R2L):
+        assert np.array_equal(gpu_scan(stream, MAX, direction, None, x, outer),
+                              O.scan(MAX, direction, None, x, outer))
+        if dt != np.float64:
+            z = dt(3)
+            assert np.array_equal(gpu_scan(stream, ADD, direction, z, x, outer),
+                                  O.scan(ADD, direction, z, x, outer))
+
+
+def test_scan_rows_float32_bound(stream):
+    outer, inner = 2000, 777
+    x = O.gen_normal_f32(5, outer * inner)
+    got = gpu_scan(stream, ADD, L2R, None, x, outer).reshape(outer, inner).astype(np.float64)
+    xr = x.reshape(outer, inner).astype(np.float64)
+    truth = np.cumsum(xr, axis=1)
+    ab = np.cumsum(np.abs(xr), axis=1)
+    terms = np.arange(1, inner + 1)  # column c sums c + 1 terms
+    bound = FP_REL * np.maximum(1.0, np.log2(np.maximum(terms, 2)))[None, :] * ab
+    assert np.all(np.abs(got - truth) <= bound + 1e-30)
+
+
 def test_scan_reference_examples(stream):
     x = np.arange(10, dtype=np.int64)
     assert gpu_scan(stream, ADD, L2R, np.int64(10), x).tolist() == [10, 10, 11, 13, 16, 20, 25, 31, 38, 46, 55]

@@ -64,8 +64,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
   const S sid{id, 0u};
   const int lane = lane_id(), w = threadIdx.x / 64;
   // the column at which a row starts / ends in traversal order
-  const int64_t c_start = REVERSE ? a.inner - 1 : 0;
-  const int64_t c_end = REVERSE ? 0 : a.inner - 1;
+  const uint32_t c_start = REVERSE ? (uint32_t)a.inner - 1 : 0u;
+  const uint32_t c_end = REVERSE ? 0u : (uint32_t)a.inner - 1;
 
   if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
   __syncthreads();
@@ -78,14 +78,44 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
                      : ((int64_t)(w * K + k) * 64 + lane);
     };
 
+    // row and column of the tile's first physical element (one 64-bit
+    // division per tile; rows are shorter than 2^20, so a vector's column
+    // offset from it fits 32 bits)
+    const int64_t brow = base / a.inner;
+    const uint32_t bcol = (uint32_t)(base - brow * a.inner);
+    const uint32_t inner32 = (uint32_t)a.inner;
+
     // ---- load; per vector: traversal-ordered values, head bits (row starts
-    // and padding), column of the first physical element
+    // and padding), row and column of the first physical element
+    // row (from brow) and column of the first physical element of vector k
+    auto vec_pos = [&](int k, uint32_t* col, int64_t* row) {
+      const uint32_t cc = bcol + (uint32_t)(vidx(k) * VEC);
+      const uint32_t dr = cc / inner32;
+      *col = cc - dr * inner32;
+      *row = brow + dr;
+    };
+    // column and row of physical slot j of a vector starting at (col, row):
+    // at most VEC - 1 wraps
+    auto slot_pos = [&](uint32_t col, int64_t row, int j, uint32_t* c, int64_t* r) {
+      uint32_t cj = col + (uint32_t)j;
+      int64_t rj = row;
+#pragma unroll
+      for (int q = 0; q < VEC - 1; ++q)
+        if (cj >= inner32) {
+          cj -= inner32;
+          ++rj;
+        }
+      *c = cj;
+      *r = rj;
+    };
     T x[K][VEC];
     uint32_t hb[K];
-    int64_t col0[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t e0 = base + vidx(k) * VEC;
+      uint32_t vcol;
+      int64_t vrow;
+      vec_pos(k, &vcol, &vrow);
       T v[VEC];
       if (ALIGNED && e0 + VEC <= a.n) {
         const Vec<T> r = load16_stream(a.in + e0);
@@ -95,14 +125,13 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
 #pragma unroll
         for (int j = 0; j < VEC; ++j) v[j] = e0 + j < a.n ? a.in[e0 + j] : id;
       }
-      const int64_t c0 = e0 % a.inner;
-      col0[k] = c0;
       uint32_t h = 0;
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
         const int pj = REVERSE ? VEC - 1 - j : j;  // physical slot of traversal slot j
-        int64_t c = c0 + pj;
-        while (c >= a.inner) c -= a.inner;
+        uint32_t c;
+        int64_t r;
+        slot_pos(vcol, vrow, pj, &c, &r);
         const bool pad = e0 + pj >= a.n;
         h |= (pad || c == c_start ? 1u : 0u) << j;
         x[k][j] = v[pj];
@@ -145,7 +174,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
       }
       // the tile's first traversal element continues a row
       const int64_t first_p = REVERSE ? base + TILE - 1 : base;
-      const bool first_is_start = first_p >= a.n || first_p % a.inner == c_start;
+      const bool first_is_start = first_p >= a.n || (uint32_t)(first_p % a.inner) == c_start;
       if (t > 0 && !first_is_start) {
         X = wave_lookback<S>(a.status, t, epoch, sop);
         if (lane == 0 && !agg.f && !last) Tagged<S>::publish(slot, sop(X, agg), make_tag(epoch, ST_PREFIX));
@@ -160,7 +189,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
     for (int k = 0; k < K; ++k) {
       const S P = sop(XW, pre[k]);  // everything before this vector, segmented
       const int64_t e0 = base + vidx(k) * VEC;
-      const int64_t row0 = e0 / a.inner;
+      uint32_t vcol;
+      int64_t vrow;
+      vec_pos(k, &vcol, &vrow);
       T y[VEC];  // by physical slot
       bool headed = false;  // a row start at or before traversal slot j in this vector
 #pragma unroll
@@ -183,11 +214,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
         // slot at its first
         const int64_t p = e0 + pj;
         if (p < a.n) {
-          int64_t c = col0[k] + pj, r = row0;
-          while (c >= a.inner) {
-            c -= a.inner;
-            ++r;
-          }
+          uint32_t c;
+          int64_t r;
+          slot_pos(vcol, vrow, pj, &c, &r);
           if (a.total_out && c == c_end) a.total_out[r * a.total_stride] = a.has_seed ? op(a.seed, incl) : incl;
           if (a.seed_out && c == c_start) a.seed_out[r * a.seed_stride] = a.seed;
         }
@@ -202,11 +231,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
         for (int j = 0; j < VEC; ++j) {
           const int64_t p = e0 + j;
           if (p < a.n) {
-            int64_t c = col0[k] + j, r = row0;
-            while (c >= a.inner) {
-              c -= a.inner;
-              ++r;
-            }
+            uint32_t c;
+            int64_t r;
+            slot_pos(vcol, vrow, j, &c, &r);
             a.out[r * a.out_row + c] = y[j];
           }
         }
@@ -222,8 +249,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
 
 template <typename T, typename Op, bool REVERSE, bool ALIGNED>
 int scan_rows_launch(acc_stream_t s, RowScanArgs<T> a) {
-  constexpr int BLOCK = 256, K = 4;
+#ifndef ACC_ROWS_K
+#define ACC_ROWS_K 4
+#endif
+  constexpr int BLOCK = 256, K = ACC_ROWS_K;
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
+  if (a.inner >= ((int64_t)1 << 20)) return fail(ACC_ERROR_UNSUPPORTED, "scan rows: row of %lld", (long long)a.inner);
   constexpr int NW = Tagged<SegV<T>>::NW;
   const int64_t ntiles = ceil_div(a.n, TILE);
   if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "scan: %lld elements", (long long)a.n);

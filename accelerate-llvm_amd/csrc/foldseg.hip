@@ -151,7 +151,14 @@ __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, in
   }
 }
 
-// One workgroup per tile.
+// One workgroup per tile.  The tile is read with coalesced dwordx4 loads
+// (all issued first), then processed in NH halves: a half's vectors are
+// written to LDS in a transposed layout, and each thread reads back one
+// contiguous run of R elements and folds it sequentially (head bits from the
+// LDS bitmap, one word per run), so the only cross-lane work is one
+// segmented scan of the run aggregates per half.  Row stride of the LDS
+// layout: R elements plus 16 bytes, which keeps the 16-byte reads of a wave
+// free of bank conflicts.
 template <typename T, typename Op, typename Starts, int BLOCK, int K, int CAP>
 __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   constexpr bool UNIFORM = std::is_same<Starts, UniformStarts>::value;
@@ -159,13 +166,24 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   constexpr int W = BLOCK / 64;
   constexpr int TILE = BLOCK * K * VEC;
   constexpr int NBW = TILE / 32 + 1;  // head bitmap: tile positions 0..TILE
+#ifndef ACC_SEG_KH
+#define ACC_SEG_KH 4
+#endif
+  constexpr int KH = ACC_SEG_KH;      // vectors per thread per part
+  constexpr int NH = K / KH;          // halves per tile
+  constexpr int R = KH * VEC;         // run: elements per thread per half
+  constexpr int HALF = BLOCK * R;
+  constexpr int RS = R + 16 / (int)sizeof(T);  // LDS row stride of a run
+  static_assert(K % KH == 0 && R <= 16, "run layout");
+  static_assert(TILE + 1 < 32767, "tile-relative starts are 16-bit");
   using S = SegV<T>;
   __shared__ uint32_t s_heads[NBW];
-  __shared__ int32_t s_rel[UNIFORM ? 1 : CAP];
+  __shared__ int16_t s_rel[UNIFORM ? 1 : CAP];  // tile-relative starts (-1 .. TILE+1)
   __shared__ int64_t s_range[2];
   __shared__ S s_wave[W];
-  __shared__ int64_t s_open_g[W];
-  __shared__ T s_open_v[W];
+  __shared__ int64_t s_open_g;
+  __shared__ T s_open_v;
+  __shared__ __align__(16) T s_data[BLOCK * RS];
   SegOp<Op> sop{Op{}};
   Op op{};
   const T id = identity<T>(op);
@@ -174,6 +192,8 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   const int w = threadIdx.x / 64;
   const uint32_t t = blockIdx.x;
   const int64_t base = (int64_t)t * TILE;
+  // position (in the tile) of vector kk of this lane in half h
+  auto vpos = [&](int h, int kk) -> int { return h * HALF + ((w * KH + kk) * 64 + lane) * VEC; };
 
   // ---- wave 0 fetches the tile's segment range and starts first, then
   // every wave issues its data loads; the staging below overlaps them
@@ -190,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   T v[K][VEC];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int64_t e0 = base + ((w * K + k) * 64 + lane) * VEC;
+    const int64_t e0 = base + vpos(k / KH, k % KH);
     if (e0 >= a.e_lo && e0 + VEC <= a.e_end) {
       const Vec<T> r = load16_stream(a.in + e0);
 #pragma unroll
@@ -206,6 +226,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   // the elements after the last one and elements before start(0) belong to
   // "segment" -1 (no results).
   int64_t g_lo, cnt;
+  if (threadIdx.x == 0) s_open_g = -1;  // no open segment end yet
   if constexpr (UNIFORM) {
     for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
     g_lo = max(base - a.st.shift, (int64_t)0) / a.st.len;
@@ -221,7 +242,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     if (w == 0) {
       for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
       if (hdr.cnt <= SLOT && lane < hdr.cnt) {
-        s_rel[lane] = my_rel;
+        s_rel[lane] = (int16_t)my_rel;
         if (my_rel >= 0 && my_rel <= TILE) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
       }
       if (lane == 0) {
@@ -237,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
       for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
         const int64_t g = g_lo + i;
         const int32_t r = g > a.nseg ? TILE + 1 : rel_pos(a.st.start(g), base, TILE);
-        if (cnt <= CAP) s_rel[i] = r;
+        if (cnt <= CAP) s_rel[i] = (int16_t)r;
         if (r >= 0 && r <= TILE) atomicOr(&s_heads[r >> 5], 1u << (r & 31));
       }
       __syncthreads();
@@ -259,96 +280,107 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
       return g_lo + x - 1;
     }
   };
-
-  // ---- per chunk: segmented scan inside each lane's vector, across the
-  // wave, and the results at segment ends (the element before a head).  An
-  // end whose segment began before the wave's portion of the tile ("open")
-  // is stashed per wave, at most one per wave.
-  if (lane == 0) s_open_g[w] = -1;  // each wave its own slot, before its own stash
-  S run = sid;  // segmented combination of the wave's earlier chunks
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int p0 = ((w * K + k) * 64 + lane) * VEC;
-    const int64_t e0 = base + p0;
-    uint32_t invalid = 0;
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) invalid |= (e0 + j >= a.e_lo && e0 + j < a.e_end ? 0u : 1u) << j;
-    const int sh = p0 & 31;
-    uint64_t word = s_heads[p0 >> 5];
-    if (sh + VEC + 1 > 32) word |= (uint64_t)s_heads[(p0 >> 5) + 1] << 32;
-    const uint32_t b = (uint32_t)(word >> sh) & ((1u << (VEC + 1)) - 1u);
-    const uint32_t hb = (b & ((1u << VEC) - 1u)) | invalid;  // heads (padding: dead segments)
-    uint32_t eb = (b >> 1) & ~invalid;                       // ends: the next position is a head
-    T x[VEC];
-    x[0] = v[k][0];
-#pragma unroll
-    for (int j = 1; j < VEC; ++j) x[j] = (hb >> j & 1u) ? v[k][j] : op(x[j - 1], v[k][j]);
-    const S inc = wave_inclusive_scan(S{x[VEC - 1], hb ? 1u : 0u}, sop);
-    const S up = shift_up1(inc);
-    const S P = sop(run, lane > 0 ? up : sid);  // prefix of the wave's portion before this vector
-    run = sop(run, read_lane(inc, 63));
-    while (eb) {
-      const int j = __builtin_ctz(eb);
-      eb &= eb - 1u;
-      T xj = x[0];
-#pragma unroll
-      for (int jj = 1; jj < VEC; ++jj) xj = j == jj ? x[jj] : xj;
-      const bool headed = (hb & ((2u << j) - 1u)) != 0;  // a head at or before j in this vector
-      T r = headed ? xj : op(P.v, xj);
-      const int64_t g = seg_ending_at(p0 + j);
-      if (!headed && !P.f) {
-        s_open_g[w] = g;
-        s_open_v[w] = r;
-      } else if (g >= 0 && g < a.nseg) {
-        if (a.has_seed) r = op(a.seed, r);  // seed on the left
-        a.out[g] = r;
-      }
+  auto emit = [&](int64_t g, T r) {
+    if (g >= 0 && g < a.nseg) {
+      if (a.has_seed) r = op(a.seed, r);  // seed on the left
+      a.out[g] = r;
     }
-  }
-  if (lane == 0) s_wave[w] = run;
-  __syncthreads();
+  };
+  // padding positions (outside the row) exist only in a partial tile
+  const bool full = base >= a.e_lo && base + TILE <= a.e_end;
 
-  // ---- wave 0: open ends of later waves that a head earlier in the tile
-  // completes; the tile's own open end and aggregate go to the fix-up record
-  if (w == 0) {
-    S agg = sid;
+  S carry = sid;  // segmented combination of the tile's earlier halves
 #pragma unroll
-    for (int i = 0; i < W; ++i) agg = sop(agg, s_wave[i]);
-    int64_t tile_g = -1;
-    T tile_v = id;
-    if (lane < W) {
-      const int64_t g = s_open_g[lane];
-      if (g >= 0) {
-        S wp = sid;
+  for (int h = 0; h < NH; ++h) {
+    // ---- transpose the half through LDS: vector at run r, offset o
 #pragma unroll
-        for (int i = 0; i < W; ++i)
-          if (i < lane) wp = sop(wp, s_wave[i]);
-        T r = op(wp.v, s_open_v[lane]);
-        if (wp.f) {
-          if (g < a.nseg) {
-            if (a.has_seed) r = op(a.seed, r);
-            a.out[g] = r;
-          }
+    for (int kk = 0; kk < KH; ++kk) {
+      const int q0 = vpos(0, kk);  // position within the half
+      const int r = q0 / R, o = q0 - r * R;
+      Vec<T> vv;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) vv.e[j] = v[h * KH + kk][j];
+      *(Vec<T>*)&s_data[r * RS + o] = vv;
+    }
+    __syncthreads();
+    T x[R];
+#pragma unroll
+    for (int q = 0; q < R / VEC; ++q) {
+      const Vec<T> vv = *(const Vec<T>*)&s_data[threadIdx.x * RS + q * VEC];
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) x[q * VEC + j] = vv.e[j];
+    }
+    // head and end bits of the run
+    const int pos0 = h * HALF + threadIdx.x * R;
+    const uint64_t word = (uint64_t)s_heads[pos0 >> 5] | ((uint64_t)s_heads[(pos0 >> 5) + 1] << 32);
+    const uint32_t mask = (1u << R) - 1u;
+    uint32_t hb = (uint32_t)(word >> (pos0 & 31)) & mask;
+    uint32_t eb = (uint32_t)(word >> ((pos0 & 31) + 1)) & mask;
+    if (!full) {
+      uint32_t invalid = 0;
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int64_t e = base + pos0 + i;
+        invalid |= (e >= a.e_lo && e < a.e_end ? 0u : 1u) << i;
+      }
+      hb |= invalid;
+      eb &= ~invalid;
+    }
+    // ---- sequential segmented fold of the run; ends after a head in the
+    // run are final; the first end before any head ("run-open") waits for
+    // the prefix of the runs before it
+    T acc = id;
+    bool hd = false;
+    int64_t ro_g = -1;
+    T ro_v = id;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const bool hi = hb >> i & 1u;
+      acc = hi ? x[i] : op(acc, x[i]);
+      hd = hd || hi;
+      if (eb >> i & 1u) {
+        const int64_t g = seg_ending_at(pos0 + i);
+        if (hd) {
+          emit(g, acc);
         } else {
-          tile_g = g;  // no head before it in the tile: the tile's open end
-          tile_v = r;
+          ro_g = g;
+          ro_v = acc;
         }
       }
     }
-    // at most one lane holds the tile's open end
-    const uint64_t m = __ballot(tile_g >= 0);
-    if (m) {
-      const int src = __builtin_ctzll(m);
-      tile_g = read_lane(tile_g, src);
-      tile_v = read_lane(tile_v, src);
+    // ---- segmented exclusive scan of the run aggregates across the block
+    const S inc = wave_inclusive_scan(S{acc, hd ? 1u : 0u}, sop);
+    const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
+    if (lane == 63) s_wave[w] = inc;
+    __syncthreads();
+    S wp = carry, tot = carry;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const S x2 = s_wave[i];
+      if (i < w) wp = sop(wp, x2);
+      tot = sop(tot, x2);
     }
-    if (lane == 0) {
-      TileCarry<T> c;
-      c.agg = agg;
-      c.open_v = tile_v;
-      c.open_g = m ? tile_g : -1;
-      a.carry[t] = c;
+    const S P = sop(wp, lane > 0 ? up : sid);  // everything in the tile before this run
+    if (ro_g >= 0) {
+      const T r = op(P.v, ro_v);
+      if (P.f) {
+        emit(ro_g, r);
+      } else {  // no head in the tile before it: the tile's one open end
+        s_open_g = ro_g;
+        s_open_v = r;
+      }
     }
+    carry = tot;
+    __syncthreads();  // s_data and s_wave are reused by the next half
+  }
+
+  // ---- the tile's record for the fix-up: aggregate and open end
+  if (threadIdx.x == 0) {
+    TileCarry<T> c;
+    c.agg = carry;
+    c.open_g = s_open_g;
+    c.open_v = c.open_g >= 0 ? s_open_v : id;
+    a.carry[t] = c;
   }
 }
 
@@ -420,7 +452,10 @@ __global__ __launch_bounds__(BLOCK) void k_seg_fix(const TileCarry<T>* carry, ui
 template <typename T> struct SegTune {
   static constexpr int BLOCK = 256;
   static constexpr int K = ACC_SEG_K;  // dwordx4 vectors per thread
-  static constexpr int CAP = 1024;     // starts staged in LDS (beyond: searched in HBM)
+#ifndef ACC_SEG_CAP
+#define ACC_SEG_CAP 512
+#endif
+  static constexpr int CAP = ACC_SEG_CAP;  // starts staged in LDS (beyond: searched in HBM)
 };
 
 template <typename T, typename Op, typename Starts>

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
         Vec<T> r;
 #pragma unroll
         for (int j = 0; j < VEC; ++j) r.e[j] = y[j];
-        store16(a.out + e0, r);
+        store16_stream(a.out + e0, r);  // written once: non-temporal
       } else {
 #pragma unroll
         for (int j = 0; j < VEC; ++j) {

@@ -150,7 +150,7 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
       Vec<T> r;
 #pragma unroll
       for (int j = 0; j < VEC; ++j) r.e[j] = y[REVERSE ? VEC - 1 - j : j];
-      store16(a.out + e0, r);
+      store16_stream(a.out + e0, r);  // written once: non-temporal
     } else if (vk) {
 #pragma unroll
       for (int j = 0; j < VEC; ++j)

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
         Vec<T> r;
 #pragma unroll
         for (int j = 0; j < Vec<T>::N; ++j) r.e[j] = acc[oy][q * Vec<T>::N + j];
-        store16(o + q * Vec<T>::N, r);
+        store16_stream(o + q * Vec<T>::N, r);  // written once: non-temporal
       }
     } else if (gy < a.h) {
 #pragma unroll

@@ -325,7 +325,8 @@ def measure_traffic():
             name = row["Kernel_Name"].split("<")[0].split("(")[0].split("::")[-1]
             if name in seg:
                 seg[name].append(float(row["Counter_Value"]))
-            elif "copyBuffer" in row["Kernel_Name"] and int(row["Grid_Size"]) >= 65536:
+            elif ("k_copy16" in row["Kernel_Name"] or "copyBuffer" in row["Kernel_Name"]) and \
+                    int(row["Grid_Size"]) >= 65536:
                 cp.append(float(row["Counter_Value"]))
         shutil.rmtree(d, ignore_errors=True)
         if not all(seg.values()) or not cp:

@@ -546,3 +546,18 @@ def test_stencil_float64_special(stream):
     out = A.DeviceArray(81, np.float64)
     A.stencil2d(stream, 2, 2, wt, CONSTANT, 0.0, dev(stream, imp), 9, 9, out)
     assert np.array_equal(out.to_numpy(stream).reshape(9, 9)[2:7, 2:7], wt[::-1, ::-1])
+
+
+# ------------------------------------------------------------------- runtime
+@pytest.mark.parametrize("nbytes", [16, 4096, (1 << 20) + 16, (1 << 24) + 48, 17, 1000003])
+@pytest.mark.parametrize("offset", [0, 4])
+def test_device_copy(stream, nbytes, offset):
+    # 16-byte aligned whole vectors take the streaming copy kernel; the rest
+    # the runtime's copy
+    src = np.random.default_rng(nbytes).integers(0, 256, nbytes + 16, dtype=np.uint8)
+    d_src = dev(stream, src)
+    d_dst = A.DeviceArray(nbytes + 16, np.uint8)
+    A.check(A.lib().acc_hip_memcpy_d2d_async(stream.handle, A.c_void_p(d_dst.ptr + offset),
+                                             A.c_void_p(d_src.ptr + offset), nbytes))
+    got = d_dst.to_numpy(stream)
+    assert np.array_equal(got[offset:offset + nbytes], src[offset:offset + nbytes])

@@ -179,7 +179,6 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   using S = SegV<T>;
   __shared__ uint32_t s_heads[NBW];
   __shared__ int16_t s_rel[UNIFORM ? 1 : CAP];  // tile-relative starts (-1 .. TILE+1)
-  __shared__ int64_t s_range[2];
   __shared__ S s_wave[W];
   __shared__ int64_t s_open_g;
   __shared__ T s_open_v;
@@ -195,30 +194,27 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   // position (in the tile) of vector kk of this lane in half h
   auto vpos = [&](int h, int kk) -> int { return h * HALF + ((w * KH + kk) * 64 + lane) * VEC; };
 
-  // ---- wave 0 fetches the tile's segment range and starts first, then
-  // every wave issues its data loads; the staging below overlaps them
-  // (barriers wait only for LDS traffic)
+  // ---- every wave fetches the tile's segment range (wave 0 also its first
+  // SLOT starts), then issues its data loads: one dwordx4 per vector, so the
+  // loads stay straight-line and each later wait names exactly the loads it
+  // needs.  A vector outside the row re-reads the block holding the row's
+  // last element (an aligned 16-byte block with a valid element never
+  // crosses a page); the run below turns padding into the identity.  The
+  // staging overlaps the loads (barriers wait only for LDS traffic).
   TileSegs hdr{0, 0};
   int32_t my_rel = 0;
   if constexpr (!UNIFORM) {
-    if (w == 0) {
-      hdr = a.st.segs[t];
-      my_rel = a.st.slots[(size_t)t * SLOT + lane];
-    }
+    hdr = a.st.segs[t];
+    my_rel = a.st.slots[(size_t)t * SLOT + lane];
   }
-  // padding positions hold the identity and start a dead segment
   T v[K][VEC];
+  const int64_t blk_hi = (a.e_end - 1) / VEC * VEC;  // block of the row's last element
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int64_t e0 = base + vpos(k / KH, k % KH);
-    if (e0 >= a.e_lo && e0 + VEC <= a.e_end) {
-      const Vec<T> r = load16_stream(a.in + e0);
+    const int64_t e0 = min(base + vpos(k / KH, k % KH), blk_hi);
+    const Vec<T> r = load16_stream(a.in + e0);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) v[k][j] = r.e[j];
-    } else {
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) v[k][j] = (e0 + j >= a.e_lo && e0 + j < a.e_end) ? a.in[e0 + j] : id;
-    }
+    for (int j = 0; j < VEC; ++j) v[k][j] = r.e[j];
   }
 
   // ---- segment heads: every start in [base, base + TILE] sets its bit.
@@ -239,47 +235,29 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     }
     __syncthreads();
   } else {
-    if (w == 0) {
-      for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
-      if (hdr.cnt <= SLOT && lane < hdr.cnt) {
-        s_rel[lane] = (int16_t)my_rel;
-        if (my_rel >= 0 && my_rel <= TILE) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
+    g_lo = hdr.g_lo;
+    cnt = hdr.cnt;
+    if (cnt <= SLOT) {
+      if (w == 0) {
+        for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
+        if (lane < cnt) {
+          s_rel[lane] = (int16_t)my_rel;
+          if (my_rel >= 0 && my_rel <= TILE) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
+        }
       }
-      if (lane == 0) {
-        s_range[0] = hdr.g_lo;
-        s_range[1] = hdr.cnt;
-      }
-    }
-    __syncthreads();
-    g_lo = s_range[0];
-    cnt = s_range[1];
-    if (cnt > SLOT) {
+    } else {
       // many segments start in this tile: stage them from the offsets
+      for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
+      __syncthreads();
       for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
         const int64_t g = g_lo + i;
         const int32_t r = g > a.nseg ? TILE + 1 : rel_pos(a.st.start(g), base, TILE);
         if (cnt <= CAP) s_rel[i] = (int16_t)r;
         if (r >= 0 && r <= TILE) atomicOr(&s_heads[r >> 5], 1u << (r & 31));
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
-  // the segment whose last element is at tile position p
-  auto seg_ending_at = [&](int p) -> int64_t {
-    if constexpr (UNIFORM) {
-      return (base + p - a.st.shift) / a.st.len;
-    } else {
-      // smallest index i with start(g_lo + i) > p, minus one
-      int64_t x = 0, y = cnt;
-      while (x < y) {
-        const int64_t m = (x + y) >> 1;
-        const int64_t sm = cnt <= CAP ? (int64_t)s_rel[m]
-                                      : (g_lo + m > a.nseg ? (int64_t)TILE + 1 : a.st.start(g_lo + m) - base);
-        if (sm <= p) x = m + 1; else y = m;
-      }
-      return g_lo + x - 1;
-    }
-  };
   auto emit = [&](int64_t g, T r) {
     if (g >= 0 && g < a.nseg) {
       if (a.has_seed) r = op(a.seed, r);  // seed on the left
@@ -289,90 +267,121 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   // padding positions (outside the row) exist only in a partial tile
   const bool full = base >= a.e_lo && base + TILE <= a.e_end;
 
-  S carry = sid;  // segmented combination of the tile's earlier halves
+  // The halves, in one of two copies: starts searched in LDS (at most CAP
+  // starts in the tile, the common case) or in the offsets.  Separate copies
+  // keep the offsets' loads out of the LDS copy, where a wait on them would
+  // also wait for the data loads still in flight.
+  auto halves = [&](auto in_hbm) __attribute__((always_inline)) {
+    constexpr bool HBM = decltype(in_hbm)::value;
+    // the segment whose last element is at tile position p
+    auto seg_ending_at = [&](int p) -> int64_t {
+      if constexpr (UNIFORM) {
+        return (base + p - a.st.shift) / a.st.len;
+      } else {
+        // smallest index i with start(g_lo + i) > p, minus one
+        int64_t x = 0, y = cnt;
+        while (x < y) {
+          const int64_t m = (x + y) >> 1;
+          int64_t sm;
+          if constexpr (HBM) sm = g_lo + m > a.nseg ? (int64_t)TILE + 1 : a.st.start(g_lo + m) - base;
+          else sm = (int64_t)s_rel[m];
+          if (sm <= p) x = m + 1; else y = m;
+        }
+        return g_lo + x - 1;
+      }
+    };
+    S carry = sid;  // segmented combination of the tile's earlier halves
 #pragma unroll
-  for (int h = 0; h < NH; ++h) {
-    // ---- transpose the half through LDS: vector at run r, offset o
+    for (int h = 0; h < NH; ++h) {
+      // ---- transpose the half through LDS: vector at run r, offset o
 #pragma unroll
-    for (int kk = 0; kk < KH; ++kk) {
-      const int q0 = vpos(0, kk);  // position within the half
-      const int r = q0 / R, o = q0 - r * R;
-      Vec<T> vv;
+      for (int kk = 0; kk < KH; ++kk) {
+        const int q0 = vpos(0, kk);  // position within the half
+        const int r = q0 / R, o = q0 - r * R;
+        Vec<T> vv;
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) vv.e[j] = v[h * KH + kk][j];
-      *(Vec<T>*)&s_data[r * RS + o] = vv;
-    }
-    __syncthreads();
-    T x[R];
+        for (int j = 0; j < VEC; ++j) vv.e[j] = v[h * KH + kk][j];
+        *(Vec<T>*)&s_data[r * RS + o] = vv;
+      }
+      __syncthreads();
+      T x[R];
 #pragma unroll
-    for (int q = 0; q < R / VEC; ++q) {
-      const Vec<T> vv = *(const Vec<T>*)&s_data[threadIdx.x * RS + q * VEC];
+      for (int q = 0; q < R / VEC; ++q) {
+        const Vec<T> vv = *(const Vec<T>*)&s_data[threadIdx.x * RS + q * VEC];
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) x[q * VEC + j] = vv.e[j];
-    }
-    // head and end bits of the run
-    const int pos0 = h * HALF + threadIdx.x * R;
-    const uint64_t word = (uint64_t)s_heads[pos0 >> 5] | ((uint64_t)s_heads[(pos0 >> 5) + 1] << 32);
-    const uint32_t mask = (1u << R) - 1u;
-    uint32_t hb = (uint32_t)(word >> (pos0 & 31)) & mask;
-    uint32_t eb = (uint32_t)(word >> ((pos0 & 31) + 1)) & mask;
-    if (!full) {
-      uint32_t invalid = 0;
+        for (int j = 0; j < VEC; ++j) x[q * VEC + j] = vv.e[j];
+      }
+      // head and end bits of the run
+      const int pos0 = h * HALF + threadIdx.x * R;
+      const uint64_t word = (uint64_t)s_heads[pos0 >> 5] | ((uint64_t)s_heads[(pos0 >> 5) + 1] << 32);
+      const uint32_t mask = (1u << R) - 1u;
+      uint32_t hb = (uint32_t)(word >> (pos0 & 31)) & mask;
+      uint32_t eb = (uint32_t)(word >> ((pos0 & 31) + 1)) & mask;
+      if (!full) {
+        // padding: the identity, each position a head of its own dead segment
+        uint32_t invalid = 0;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          const int64_t e = base + pos0 + i;
+          const bool bad = !(e >= a.e_lo && e < a.e_end);
+          invalid |= (bad ? 1u : 0u) << i;
+          x[i] = bad ? id : x[i];
+        }
+        hb |= invalid;
+        eb &= ~invalid;
+      }
+      // ---- sequential segmented fold of the run; ends after a head in the
+      // run are final; the first end before any head ("run-open") waits for
+      // the prefix of the runs before it
+      T acc = id;
+      bool hd = false;
+      int64_t ro_g = -1;
+      T ro_v = id;
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        const int64_t e = base + pos0 + i;
-        invalid |= (e >= a.e_lo && e < a.e_end ? 0u : 1u) << i;
-      }
-      hb |= invalid;
-      eb &= ~invalid;
-    }
-    // ---- sequential segmented fold of the run; ends after a head in the
-    // run are final; the first end before any head ("run-open") waits for
-    // the prefix of the runs before it
-    T acc = id;
-    bool hd = false;
-    int64_t ro_g = -1;
-    T ro_v = id;
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const bool hi = hb >> i & 1u;
-      acc = hi ? x[i] : op(acc, x[i]);
-      hd = hd || hi;
-      if (eb >> i & 1u) {
-        const int64_t g = seg_ending_at(pos0 + i);
-        if (hd) {
-          emit(g, acc);
-        } else {
-          ro_g = g;
-          ro_v = acc;
+        const bool hi = hb >> i & 1u;
+        acc = hi ? x[i] : op(acc, x[i]);
+        hd = hd || hi;
+        if (eb >> i & 1u) {
+          const int64_t g = seg_ending_at(pos0 + i);
+          if (hd) {
+            emit(g, acc);
+          } else {
+            ro_g = g;
+            ro_v = acc;
+          }
         }
       }
-    }
-    // ---- segmented exclusive scan of the run aggregates across the block
-    const S inc = wave_inclusive_scan(S{acc, hd ? 1u : 0u}, sop);
-    const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
-    if (lane == 63) s_wave[w] = inc;
-    __syncthreads();
-    S wp = carry, tot = carry;
+      // ---- segmented exclusive scan of the run aggregates across the block
+      const S inc = wave_inclusive_scan(S{acc, hd ? 1u : 0u}, sop);
+      const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
+      if (lane == 63) s_wave[w] = inc;
+      __syncthreads();
+      S wp = carry, tot = carry;
 #pragma unroll
-    for (int i = 0; i < W; ++i) {
-      const S x2 = s_wave[i];
-      if (i < w) wp = sop(wp, x2);
-      tot = sop(tot, x2);
-    }
-    const S P = sop(wp, lane > 0 ? up : sid);  // everything in the tile before this run
-    if (ro_g >= 0) {
-      const T r = op(P.v, ro_v);
-      if (P.f) {
-        emit(ro_g, r);
-      } else {  // no head in the tile before it: the tile's one open end
-        s_open_g = ro_g;
-        s_open_v = r;
+      for (int i = 0; i < W; ++i) {
+        const S x2 = s_wave[i];
+        if (i < w) wp = sop(wp, x2);
+        tot = sop(tot, x2);
       }
+      const S P = sop(wp, lane > 0 ? up : sid);  // everything in the tile before this run
+      if (ro_g >= 0) {
+        const T r = op(P.v, ro_v);
+        if (P.f) {
+          emit(ro_g, r);
+        } else {  // no head in the tile before it: the tile's one open end
+          s_open_g = ro_g;
+          s_open_v = r;
+        }
+      }
+      carry = tot;
+      __syncthreads();  // s_data and s_wave are reused by the next half
     }
-    carry = tot;
-    __syncthreads();  // s_data and s_wave are reused by the next half
-  }
+    return carry;
+  };
+  S carry;
+  if (UNIFORM || cnt <= CAP) carry = halves(std::false_type{});
+  else carry = halves(std::true_type{});
 
   // ---- the tile's record for the fix-up: aggregate and open end
   if (threadIdx.x == 0) {

@@ -71,12 +71,13 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
   }
   auto ok = [&](int k, int j) -> bool { return !MASKED || ((valid >> (k * VEC + j)) & 1u); };
 
-  // ---- load, thread-level inclusive scan of each vector (traversal order)
+  // ---- load every vector first (so all of the tile's loads are in flight
+  // together), then the thread-level inclusive scan of each vector
+  // (traversal order)
   T x[K][VEC];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int64_t e0 = base + vidx(k) * VEC;
-    T v[VEC];
     if (ALIGNED) {
       // In the ragged tile a vector past the end re-reads the vector holding
       // the last element (an aligned 16-byte block with a valid element never
@@ -84,11 +85,19 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
       const int64_t ea = MASKED ? min(e0, ((a.n - 1) / VEC) * VEC) : e0;
       const Vec<T> r = load16_stream(a.in + ea);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) v[j] = ok(k, j) ? r.e[j] : id;
+      for (int j = 0; j < VEC; ++j) x[k][j] = r.e[j];
     } else {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) v[j] = ok(k, j) ? a.in[e0 + j] : id;
+      for (int j = 0; j < VEC; ++j) x[k][j] = ok(k, j) ? a.in[e0 + j] : id;
     }
+  }
+  // keep the loads above ahead of the arithmetic below
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    T v[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) v[j] = ALIGNED && !ok(k, j) ? id : x[k][j];
 #pragma unroll
     for (int j = 0; j < VEC; ++j) x[k][j] = v[REVERSE ? VEC - 1 - j : j];
 #pragma unroll

@@ -111,18 +111,37 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   }
   const int64_t x0 = (int64_t)tx * ST_TW, y0 = (int64_t)ty * ST_TH;
 
-  // ---- stage the halo tile
+  // ---- stage the halo tile.  Every load of the tile is issued before the
+  // first LDS write (indexes past the tile re-read its last element, so the
+  // loads need no branch and each wait names only the loads it needs).
   if constexpr (INTERIOR) {
     constexpr int NV = LS / VEC;  // vectors per LDS row
+    constexpr int TOT = LH * NV;
+    constexpr int NI = (TOT + ST_BLOCK - 1) / ST_BLOCK;
     const T* src = a.in + (y0 - RY) * a.w + (x0 - PADL);
-    for (int i = threadIdx.x; i < LH * NV; i += ST_BLOCK) {
+    Vec<T> vv[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int i = min((int)threadIdx.x + u * ST_BLOCK, TOT - 1);
       const int r = i / NV, q = i - r * NV;
-      const Vec<T> v = load16(src + (int64_t)r * a.w + q * VEC);
-      *(Vec<T>*)&tile[r * LS + q * VEC] = v;
+      vv[u] = load16(src + (int64_t)r * a.w + q * VEC);
+    }
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int i = threadIdx.x + u * ST_BLOCK;
+      const int r = i / NV, q = i - r * NV;
+      if (i < TOT) *(Vec<T>*)&tile[r * LS + q * VEC] = vv[u];
     }
   } else {
+    constexpr int TOT = LH * LW;
+    constexpr int NI = (TOT + ST_BLOCK - 1) / ST_BLOCK;
     const bool constant = a.boundary == ACC_BOUNDARY_CONSTANT;
-    for (int i = threadIdx.x; i < LH * LW; i += ST_BLOCK) {
+    T vb[NI];
+    uint32_t outside = 0;  // bit u: element u of this thread lies outside the array
+    static_assert(NI <= 32, "halo tile too large for the outside mask");
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int i = min((int)threadIdx.x + u * ST_BLOCK, TOT - 1);
       const int r = i / LW, c = i - r * LW;
       int64_t gy = y0 - RY + r, gx = x0 - RX + c;
       const bool inside = (uint64_t)gy < (uint64_t)a.h && (uint64_t)gx < (uint64_t)a.w;
@@ -130,8 +149,15 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
         gy = bound_index(a.boundary, gy, a.h);
         gx = bound_index(a.boundary, gx, a.w);
       }
-      T v = a.in[gy * a.w + gx];
-      tile[r * LS + c + CO] = (inside || !constant) ? v : a.cval;
+      outside |= (inside ? 0u : 1u) << u;
+      vb[u] = a.in[gy * a.w + gx];
+    }
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int i = threadIdx.x + u * ST_BLOCK;
+      const int r = i / LW, c = i - r * LW;
+      const bool use_cval = constant && (outside >> u & 1u);
+      if (i < TOT) tile[r * LS + c + CO] = use_cval ? a.cval : vb[u];
     }
   }
   __syncthreads();

@@ -81,12 +81,20 @@ __device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, 
       for (int j = 0; j < VEC; ++j) r = op(r, v[k].e[j]);
     acc = r;
   } else {
+    // the problem's last tile: a vector past the end re-reads the vector
+    // holding the last element (an aligned 16-byte block with a valid element
+    // never crosses a page), so every load is one unconditional dwordx4
+    Vec<T> v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
+      v[k] = load16_stream(in + min(e0, (d.n - 1) / VEC * VEC));
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
 #pragma unroll
-      for (int j = 0; j < VEC; ++j)
-        acc = op(acc, e0 + j < d.n ? in[e0 + j] : id);
+      for (int j = 0; j < VEC; ++j) acc = op(acc, e0 + j < d.n ? v[k].e[j] : id);
     }
   }
   const T wr = wave_reduce(acc, op);

@@ -167,7 +167,9 @@ __global__ __launch_bounds__(BLOCK) void k_permute_lds(PermArgs<T, I> a) {
   };
 
   // U index vectors (and their source values) in flight per thread before
-  // the LDS updates: one load at a time leaves the loop latency-bound
+  // the LDS updates: one load at a time leaves the loop latency-bound.  The
+  // loads are unconditional (a vector past the chunk re-reads its last one)
+  // so that each wait names only the loads it needs; the updates skip them.
   constexpr int U = CONST_SRC ? 8 : 4;
   const I* body = a.idx + a.head;
   const int64_t vb = (int64_t)blockIdx.x * a.chunk;
@@ -177,13 +179,11 @@ __global__ __launch_bounds__(BLOCK) void k_permute_lds(PermArgs<T, I> a) {
     T sv[U][VI];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int64_t i = i0 + (int64_t)u * BLOCK;
-      if (i < ve) {
-        iv[u] = load16_stream(body + i * VI);
-        if (!CONST_SRC) {
+      const int64_t i = min(i0 + (int64_t)u * BLOCK, ve - 1);
+      iv[u] = load16_stream(body + i * VI);
+      if (!CONST_SRC) {
 #pragma unroll
-          for (int k = 0; k < VI; ++k) sv[u][k] = a.src[a.head + i * VI + k];
-        }
+        for (int k = 0; k < VI; ++k) sv[u][k] = a.src[a.head + i * VI + k];
       }
     }
 #pragma unroll

@@ -110,6 +110,24 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
     };
     T x[K][VEC];
     uint32_t hb[K];
+    // every load of the tile first, so they are all in flight together; a
+    // vector past the end re-reads the vector holding the last element (an
+    // aligned 16-byte block with a valid element never crosses a page)
+    T raw[K][VEC];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t e0 = base + vidx(k) * VEC;
+      if (ALIGNED) {
+        const Vec<T> r = load16_stream(a.in + min(e0, (a.n - 1) / VEC * VEC));
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) raw[k][j] = r.e[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) raw[k][j] = e0 + j < a.n ? a.in[e0 + j] : id;
+      }
+    }
+    // keep the loads above ahead of the arithmetic below
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t e0 = base + vidx(k) * VEC;
@@ -117,14 +135,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
       int64_t vrow;
       vec_pos(k, &vcol, &vrow);
       T v[VEC];
-      if (ALIGNED && e0 + VEC <= a.n) {
-        const Vec<T> r = load16_stream(a.in + e0);
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) v[j] = r.e[j];
-      } else {
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) v[j] = e0 + j < a.n ? a.in[e0 + j] : id;
-      }
+      for (int j = 0; j < VEC; ++j) v[j] = e0 + j < a.n ? raw[k][j] : id;
       uint32_t h = 0;
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
@@ -239,6 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
         }
       }
     }
+    if (gridDim.x >= a.ntiles) break;  // one tile per workgroup
     __syncthreads();
     if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
     __syncthreads();
@@ -250,7 +263,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
 template <typename T, typename Op, bool REVERSE, bool ALIGNED>
 int scan_rows_launch(acc_stream_t s, RowScanArgs<T> a) {
 #ifndef ACC_ROWS_K
-#define ACC_ROWS_K 4
+#define ACC_ROWS_K 8
 #endif
   constexpr int BLOCK = 256, K = ACC_ROWS_K;
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;

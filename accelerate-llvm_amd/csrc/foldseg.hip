@@ -59,11 +59,29 @@ struct TileSegs {
   int64_t cnt;
 };
 
-// Per-tile record for k_seg_fix.
+// Per-tile record for the fix-up.
 template <typename T> struct TileCarry {
   SegV<T> agg;     // segmented combination of the whole tile
   T open_v;        // the tile's first elements up to the end of their segment
   int64_t open_g;  // that segment, or -1 when the tile has no open end
+};
+
+// The records as four arrays (every array 256-byte aligned), so that the
+// fix-up's reads of consecutive tiles' records coalesce.
+template <typename T> struct CarryArrays {
+  T* agg_v;
+  uint8_t* agg_f;
+  T* open_v;
+  int64_t* open_g;
+  __device__ __forceinline__ TileCarry<T> get(uint32_t t) const {
+    return TileCarry<T>{SegV<T>{agg_v[t], (uint32_t)agg_f[t]}, open_v[t], open_g[t]};
+  }
+  __device__ __forceinline__ void put(uint32_t t, const TileCarry<T>& c) const {
+    agg_v[t] = c.agg.v;
+    agg_f[t] = c.agg.f ? 1 : 0;
+    open_v[t] = c.open_v;
+    open_g[t] = c.open_g;
+  }
 };
 
 // Segment starts given by an offsets array: start(g) = off[g] + shift, g in [0, S].
@@ -92,7 +110,7 @@ template <typename T, typename Starts> struct SegArgs {
   uint32_t ntiles;
   uint32_t has_seed;
   T seed;
-  TileCarry<T>* carry;
+  CarryArrays<T> carry;
 };
 
 // A tile-relative position clamped to [-1, tile + 1]: order against the
@@ -389,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     c.agg = carry;
     c.open_g = s_open_g;
     c.open_v = c.open_g >= 0 ? s_open_v : id;
-    a.carry[t] = c;
+    a.carry.put(t, c);
   }
 }
 
@@ -400,7 +418,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
 // carry from the workgroups before it with a decoupled look-back (with at
 // most 64 workgroups one window covers them all).
 template <typename T, typename Op, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_seg_fix(const TileCarry<T>* carry, uint32_t ntiles, T* out,
+__global__ __launch_bounds__(BLOCK) void k_seg_fix(CarryArrays<T> carry, uint32_t ntiles, T* out,
                                                    int64_t nseg, uint32_t has_seed, T seed, SyncCtl* ctl,
                                                    uint64_t* status) {
   constexpr int W = BLOCK / 64;
@@ -419,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_fix(const TileCarry<T>* carry, ui
   const uint32_t b = (uint32_t)s_tick;  // < gridDim.x: one ticket per workgroup
   const uint32_t t = b * BLOCK + threadIdx.x;
   TileCarry<T> c;
-  if (t < ntiles) c = carry[t];
+  if (t < ntiles) c = carry.get(t);
   else c = TileCarry<T>{sid, sid.v, -1};
   const S inc = wave_inclusive_scan(c.agg, sop);
   const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
@@ -487,7 +505,10 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
   // workspace: per-tile carry records, then (offsets) segment ranges and starts
   SyncCtl* ctl;
   uint64_t* ws;
-  const size_t carry_bytes = ((size_t)ntiles * sizeof(TileCarry<T>) + 255) / 256 * 256;
+  auto al256 = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t cv_bytes = al256((size_t)ntiles * sizeof(T)), cf_bytes = al256((size_t)ntiles);
+  const size_t cg_bytes = (size_t)ntiles * 8;
+  const size_t carry_bytes = al256(2 * cv_bytes + cf_bytes + cg_bytes);
   const size_t segs_bytes = offsets ? ((size_t)ntiles * sizeof(TileSegs) + 255) / 256 * 256 : 0;
   const size_t slot_bytes = offsets ? (size_t)ntiles * SLOT * sizeof(int32_t) : 0;
   constexpr int FIX = 256;  // tiles per k_seg_fix workgroup
@@ -495,7 +516,9 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
   const size_t fix_bytes = (size_t)nfix * Tagged<SegV<T>>::NW * 8;
   ACC_CHECK(stream_workspace(s, carry_bytes + segs_bytes + slot_bytes + fix_bytes, &ctl, &ws));
   uint64_t* fix_status = (uint64_t*)((char*)ws + carry_bytes + segs_bytes + slot_bytes);
-  TileCarry<T>* carry = (TileCarry<T>*)ws;
+  char* cb = (char*)ws;
+  const CarryArrays<T> carry{(T*)cb, (uint8_t*)(cb + cv_bytes), (T*)(cb + cv_bytes + cf_bytes),
+                             (int64_t*)(cb + 2 * cv_bytes + cf_bytes)};
   const bool fill_empty = seed && check_empty;
   if constexpr (!std::is_same<Starts, UniformStarts>::value) {
     TileSegs* segs = (TileSegs*)((char*)ws + carry_bytes);
@@ -518,9 +541,8 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
                      a);
   ACC_HIP_CHECK(hipGetLastError());
   if (ntiles > 1) {
-    hipLaunchKernelGGL((k_seg_fix<T, Op, FIX>), dim3((unsigned)nfix), dim3(FIX), 0, s->stream,
-                       (const TileCarry<T>*)carry, (uint32_t)ntiles, out, nseg, seed ? 1u : 0u, seed ? *seed : T(),
-                       ctl, fix_status);
+    hipLaunchKernelGGL((k_seg_fix<T, Op, FIX>), dim3((unsigned)nfix), dim3(FIX), 0, s->stream, carry,
+                       (uint32_t)ntiles, out, nseg, seed ? 1u : 0u, seed ? *seed : T(), ctl, fix_status);
     ACC_HIP_CHECK(hipGetLastError());
   }
   return ACC_SUCCESS;

@@ -4,11 +4,11 @@
 // (permute_rmw, PTX/CodeGen/Permute.hs:117-213), so every update of a hot
 // output slot (a Zipf histogram's top bin takes ~16% of them) serialises at
 // the L2 of one channel.  Here, when the output fits in LDS, each workgroup
-// keeps a private copy of the output in LDS, folds its share of the source
-// into it with LDS atomics (for a constant source with (+), lanes of a wave
-// that hit the same slot are first aggregated with a ballot), and flushes the
-// non-neutral slots to the output with one global atomic each.  Larger
-// outputs take one global atomic per element, as the reference does.
+// keeps private copies of the output in LDS (several, so that lanes hitting
+// one hot slot spread over several addresses), folds its share of the source
+// into them with LDS atomics, and flushes the merged non-neutral slots to the
+// output with one global atomic each.  Larger outputs take one global atomic
+// per element, as the reference does.
 //
 // Combine functions are those the reference turns into atomics
 // (LLVM/CodeGen/Permute.hs:117-145): new (+) old for + min max and or xor,
@@ -49,8 +49,9 @@ __device__ __forceinline__ void cas_loop(T* p, T v, F f) {
 }
 
 // Each Rmw has: identity() (neutral for LDS privatisation, when one exists),
-// apply(p, v) at agent scope (global) and lds(p, v) at workgroup scope, and
-// flush(p, v) applying a private partial to the output.
+// apply(p, v) at agent scope (global) and lds(p, v) at workgroup scope,
+// flush(p, v) applying a private partial to the output, and merge(x, y)
+// combining two private partials of one slot.
 template <typename T> struct RmwAdd {
   static constexpr bool privatise = true;
   static __device__ __forceinline__ T identity() { return T(0); }
@@ -63,6 +64,7 @@ template <typename T> struct RmwAdd {
     else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+  static __device__ __forceinline__ T merge(T x, T y) { return x + y; }
 };
 
 // old - new: privatised as a sum of the new values, flushed by subtraction
@@ -75,6 +77,7 @@ template <typename T> struct RmwSub {
   }
   static __device__ __forceinline__ void lds(T* p, T v) { RmwAdd<T>::lds(p, v); }
   static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+  static __device__ __forceinline__ T merge(T x, T y) { return x + y; }
 };
 
 template <typename T, bool IS_MAX> struct RmwMinMax {
@@ -100,6 +103,7 @@ template <typename T, bool IS_MAX> struct RmwMinMax {
     }
   }
   static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+  static __device__ __forceinline__ T merge(T x, T y) { return IS_MAX ? (x < y ? y : x) : (y < x ? y : x); }
 };
 
 template <typename T, int K> struct RmwBits {  // K: 0 and, 1 or, 2 xor
@@ -113,6 +117,7 @@ template <typename T, int K> struct RmwBits {  // K: 0 and, 1 or, 2 xor
   static __device__ __forceinline__ void apply(T* p, T v) { op<__HIP_MEMORY_SCOPE_AGENT>(p, v); }
   static __device__ __forceinline__ void lds(T* p, T v) { op<__HIP_MEMORY_SCOPE_WORKGROUP>(p, v); }
   static __device__ __forceinline__ void flush(T* p, T v) { apply(p, v); }
+  static __device__ __forceinline__ T merge(T x, T y) { return K == 0 ? (x & y) : K == 1 ? (x | y) : (x ^ y); }
 };
 
 template <typename T> struct RmwExchange {
@@ -121,6 +126,7 @@ template <typename T> struct RmwExchange {
   static __device__ __forceinline__ void apply(T* p, T v) { *p = v; }
   static __device__ __forceinline__ void lds(T*, T) {}
   static __device__ __forceinline__ void flush(T* p, T v) { *p = v; }
+  static __device__ __forceinline__ T merge(T x, T) { return x; }
 };
 
 template <typename T, typename I> struct PermArgs {
@@ -133,37 +139,37 @@ template <typename T, typename I> struct PermArgs {
   int64_t head;   // index elements before the first 16-byte aligned one
   int64_t nvec;   // whole 16-byte index vectors after the head
   int64_t chunk;  // vectors per workgroup
+  int32_t copies; // private copies of the output per workgroup (k_permute_lds)
 };
 
 template <typename I> __device__ __forceinline__ bool in_range(I j, int64_t m) {
   return (int64_t)j >= 0 && (int64_t)j < m;
 }
 
-// Private LDS copy of the whole output per workgroup.
+// Private LDS copies of the whole output per workgroup: lane l updates copy
+// l % copies, so lanes of one wave that hit the same hot slot (Zipf bins)
+// spread over several addresses, and the copy stride m + 1 puts one slot's
+// copies in different banks.  An index outside the output updates a dummy
+// word of the lane's own (after the copies) instead of branching, so an
+// update is an address select and one LDS atomic.  At the end the copies of
+// each slot are merged and flushed.
 template <typename T, typename I, typename R, bool CONST_SRC, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_permute_lds(PermArgs<T, I> a) {
   extern __shared__ __align__(16) unsigned char s_raw[];
   T* bins = (T*)s_raw;
   constexpr int VI = 16 / sizeof(I);
   const T ident = R::identity();
-  for (int64_t b = threadIdx.x; b < a.m; b += BLOCK) bins[b] = ident;
+  const int64_t stride = a.m + 1;
+  const int64_t words = stride * a.copies;  // the dummy words follow
+  for (int64_t b = threadIdx.x; b < words + 64; b += BLOCK) bins[b] = ident;
   __syncthreads();
 
+  const uint32_t m32 = (uint32_t)a.m;  // m < 2^31: the output fits in LDS
+  T* const mine = bins + (int64_t)(lane_id() % a.copies) * stride;
+  T* const dummy = bins + words + lane_id();
   auto update = [&](I j, T v) {
-    if (CONST_SRC && std::is_same<R, RmwAdd<T>>::value) {
-      // aggregate lanes of the wave that hit the same slot (hot Zipf bins)
-      const bool ok = in_range(j, a.m);
-      const I lead = (I)__builtin_amdgcn_readfirstlane((int)j);
-      const bool same = ok && j == lead;
-      const uint64_t mask = __ballot(same);
-      if (same) {
-        if (lane_id() == __builtin_ctzll(mask)) R::lds(&bins[j], (T)(__popcll(mask)) * v);
-      } else if (ok) {
-        R::lds(&bins[j], v);
-      }
-    } else {
-      if (in_range(j, a.m)) R::lds(&bins[j], v);
-    }
+    T* p = (uint64_t)(int64_t)j < (uint64_t)m32 ? mine + (int64_t)j : dummy;
+    R::lds(p, v);
   };
 
   // U index vectors (and their source values) in flight per thread before
@@ -209,7 +215,8 @@ __global__ __launch_bounds__(BLOCK) void k_permute_lds(PermArgs<T, I> a) {
   }
   __syncthreads();
   for (int64_t b = threadIdx.x; b < a.m; b += BLOCK) {
-    const T v = bins[b];
+    T v = bins[b];
+    for (int c = 1; c < a.copies; ++c) v = R::merge(v, bins[c * stride + b]);
     if (v != ident) R::flush(&a.out[b], v);
   }
 }
@@ -234,12 +241,23 @@ int permute_launch(acc_stream_t s, T* out, int64_t m, const T* src, T fill, cons
   int64_t head = (int64_t)(((16 - (uintptr_t)idx % 16) % 16) / sizeof(I));
   if (head > n) head = n;
   const int64_t nvec = (n - head) / VI;
-  PermArgs<T, I> a{idx, src, fill, n, out, m, head, nvec, 0};
-  const size_t lds = (size_t)m * sizeof(T);
-  const bool privatise = R::privatise && lds <= 64 * 1024;
+  PermArgs<T, I> a{idx, src, fill, n, out, m, head, nvec, 0, 1};
+  // private copies: as many as fit in 64 KiB of LDS (with the 64 dummy
+  // words), at most ACC_PERM_COPIES
+#ifndef ACC_PERM_COPIES
+#define ACC_PERM_COPIES 2
+#endif
+  constexpr int64_t LDS_WORDS = 64 * 1024 / sizeof(T);
+  const int64_t copies = min((int64_t)ACC_PERM_COPIES, (LDS_WORDS - 64) / (m + 1));
+  const bool privatise = R::privatise && copies >= 1;
   if (privatise) {
+    a.copies = (int32_t)copies;
+    const size_t lds = (size_t)(copies * (m + 1) + 64) * sizeof(T);
     int64_t nb = ceil_div(nvec, (int64_t)BLOCK * 8);
-    const int64_t maxb = (int64_t)di->compute_units * 2;
+#ifndef ACC_PERM_WGS
+#define ACC_PERM_WGS 2
+#endif
+    const int64_t maxb = (int64_t)di->compute_units * ACC_PERM_WGS;
     if (nb > maxb) nb = maxb;
     if (nb < 1) nb = 1;
     a.chunk = nvec ? ceil_div(nvec, nb) : 0;

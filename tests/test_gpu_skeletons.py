@@ -397,6 +397,21 @@ def test_fold_seg_starts_per_tile(stream, maxlen, itype):
         assert np.array_equal(o.to_numpy(stream), O.fold_seg(op, seed, x, off.astype(np.int64))), (op, maxlen)
 
 
+@pytest.mark.parametrize("dt,n", [(np.int32, 1 << 26), (np.int64, (1 << 25) + 4099)])
+def test_fold_seg_fixup_sizes(stream, dt, n):
+    # 8192 Int32 tiles and 8194 Int64 tiles: fix-up workgroups of 256 tiles,
+    # the last one full and one with two tiles
+    lens = O.gen_segment_lengths(41, n // 1500, 3000, n)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    nseg = off.size - 1
+    x = rand(dt, n, 42)
+    d, do = dev(stream, x), dev(stream, off)
+    o = A.DeviceArray(nseg, dt)
+    for op, seed in ((ADD, None), (MAX, None), (ADD, dt(-3))):
+        A.fold_seg(stream, op, seed, d, 1, n, do, nseg, o)
+        assert np.array_equal(o.to_numpy(stream), O.fold_seg(op, seed, x, off)), (op, seed)
+
+
 def test_fold_seg_float64(stream):
     off = make_offsets(2000, 5000, 3000000, 31)
     n = int(off[-1])

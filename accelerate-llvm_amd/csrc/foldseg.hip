@@ -197,6 +197,8 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   using S = SegV<T>;
   __shared__ uint32_t s_heads[NBW];
   __shared__ int16_t s_rel[UNIFORM ? 1 : CAP];  // tile-relative starts (-1 .. TILE+1)
+  __shared__ int16_t s_pre[UNIFORM ? 1 : NBW];  // heads in the bitmap words before each word
+  __shared__ int32_t s_nin, s_neg, s_mode;      // starts in [0, TILE]; one before 0; search mode
   __shared__ S s_wave[W];
   __shared__ int64_t s_open_g;
   __shared__ T s_open_v;
@@ -254,25 +256,64 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     __syncthreads();
   } else {
     g_lo = hdr.g_lo;
-    cnt = hdr.cnt;
+    cnt = hdr.cnt;  // >= 2
     if (cnt <= SLOT) {
       if (w == 0) {
         for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
+        const bool in = lane < cnt && my_rel >= 0 && my_rel <= TILE;
         if (lane < cnt) {
           s_rel[lane] = (int16_t)my_rel;
-          if (my_rel >= 0 && my_rel <= TILE) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
+          if (in) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
+        }
+        const uint64_t inb = __ballot(in);
+        if (lane == 0) {
+          s_nin = __popcll(inb);
+          s_neg = my_rel < 0 ? 1 : 0;
         }
       }
     } else {
       // many segments start in this tile: stage them from the offsets
       for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
+      if (threadIdx.x == 0) s_nin = 0;
       __syncthreads();
+      int my_in = 0;
       for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
         const int64_t g = g_lo + i;
         const int32_t r = g > a.nseg ? TILE + 1 : rel_pos(a.st.start(g), base, TILE);
         if (cnt <= CAP) s_rel[i] = (int16_t)r;
-        if (r >= 0 && r <= TILE) atomicOr(&s_heads[r >> 5], 1u << (r & 31));
+        if (r >= 0 && r <= TILE) {
+          atomicOr(&s_heads[r >> 5], 1u << (r & 31));
+          ++my_in;
+        }
+        if (i == 0) s_neg = r < 0 ? 1 : 0;
       }
+      if (my_in) atomicAdd(&s_nin, my_in);
+      __syncthreads();
+    }
+    // wave 0: heads before each bitmap word.  When the starts inside the
+    // tile are distinct (no empty segment begins inside it: as many heads as
+    // starts in [0, TILE]) the segment of an end is a count of heads;
+    // otherwise a binary search over the starts, in LDS or, past CAP of
+    // them, in the offsets.
+    if (w == 0) {
+      constexpr int WPL = (NBW + 63) / 64;  // bitmap words per lane
+      int c[WPL];
+      int sum = 0;
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const int wd = lane * WPL + k;
+        c[k] = wd < NBW ? __popc(s_heads[wd]) : 0;
+        sum += c[k];
+      }
+      const int incl = wave_inclusive_scan(sum, OpAdd{});
+      int run = incl - sum;
+#pragma unroll
+      for (int k = 0; k < WPL; ++k) {
+        const int wd = lane * WPL + k;
+        if (wd < NBW) s_pre[wd] = (int16_t)run;
+        run += c[k];
+      }
+      if (lane == 63) s_mode = incl == s_nin ? 0 : (cnt <= CAP ? 1 : 2);
     }
     __syncthreads();
   }
@@ -285,16 +326,25 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   // padding positions (outside the row) exist only in a partial tile
   const bool full = base >= a.e_lo && base + TILE <= a.e_end;
 
-  // The halves, in one of two copies: starts searched in LDS (at most CAP
-  // starts in the tile, the common case) or in the offsets.  Separate copies
-  // keep the offsets' loads out of the LDS copy, where a wait on them would
-  // also wait for the data loads still in flight.
-  auto halves = [&](auto in_hbm) __attribute__((always_inline)) {
-    constexpr bool HBM = decltype(in_hbm)::value;
-    // the segment whose last element is at tile position p
+  // The halves, in one copy per way of finding the segment of an end: a
+  // count of heads (MODE 0), a binary search over the starts in LDS (1) or
+  // in the offsets (2).  Separate copies keep the offsets' loads out of the
+  // others, where a wait on them would also wait for the data loads still in
+  // flight.
+  const int neg = UNIFORM ? 0 : s_neg;
+  auto halves = [&](auto mode) __attribute__((always_inline)) {
+    constexpr int MODE = decltype(mode)::value;
+    constexpr bool HBM = MODE == 2;
+    // the segment whose last element is at tile position p (0 <= p < TILE)
     auto seg_ending_at = [&](int p) -> int64_t {
       if constexpr (UNIFORM) {
         return (base + p - a.st.shift) / a.st.len;
+      } else if constexpr (MODE == 0) {
+        // starts at or before p: the one before the tile (if any) and the
+        // heads at 0 .. p
+        const int wd = p >> 5;
+        const int h = s_pre[wd] + __popc(s_heads[wd] & (0xffffffffu >> (31 - (p & 31))));
+        return g_lo - 1 + neg + h;
       } else {
         // smallest index i with start(g_lo + i) > p, minus one
         int64_t x = 0, y = cnt;
@@ -350,28 +400,37 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
       }
       // ---- sequential segmented fold of the run; ends after a head in the
       // run are final; the first end before any head ("run-open") waits for
-      // the prefix of the runs before it
+      // the prefix of the runs before it.  A half holds one or two segment
+      // boundaries per wave on average, so each position first tests the
+      // wave's own head and end bits (one OR over the wave, scalar branches):
+      // a position with no head in the wave costs the combine alone, one
+      // with no end skips the end handling.
+      static_assert(R <= 16, "head and end bits share one word");
+      const uint32_t wbits = wave_reduce(hb | eb << 16, OpOr{});
       T acc = id;
-      bool hd = false;
       int64_t ro_g = -1;
       T ro_v = id;
 #pragma unroll
       for (int i = 0; i < R; ++i) {
-        const bool hi = hb >> i & 1u;
-        acc = hi ? x[i] : op(acc, x[i]);
-        hd = hd || hi;
-        if (eb >> i & 1u) {
-          const int64_t g = seg_ending_at(pos0 + i);
-          if (hd) {
-            emit(g, acc);
-          } else {
-            ro_g = g;
-            ro_v = acc;
+        if (wbits >> i & 1u) acc = (hb >> i & 1u) ? x[i] : op(acc, x[i]);
+        else acc = op(acc, x[i]);
+        if (wbits >> (16 + i) & 1u) {
+          // (keeps this test a scalar branch: without it the two tests merge
+          // into one per-lane condition evaluated at every position)
+          __builtin_amdgcn_sched_barrier(0);
+          if (eb >> i & 1u) {
+            const int64_t g = seg_ending_at(pos0 + i);
+            if (hb & ((2u << i) - 1u)) {  // a head at or before i in the run
+              emit(g, acc);
+            } else {
+              ro_g = g;
+              ro_v = acc;
+            }
           }
         }
       }
       // ---- segmented exclusive scan of the run aggregates across the block
-      const S inc = wave_inclusive_scan(S{acc, hd ? 1u : 0u}, sop);
+      const S inc = wave_inclusive_scan(S{acc, hb != 0 ? 1u : 0u}, sop);
       const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
       if (lane == 63) s_wave[w] = inc;
       __syncthreads();
@@ -398,8 +457,14 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     return carry;
   };
   S carry;
-  if (UNIFORM || cnt <= CAP) carry = halves(std::false_type{});
-  else carry = halves(std::true_type{});
+  if constexpr (UNIFORM) {
+    carry = halves(std::integral_constant<int, 0>{});
+  } else {
+    const int m = s_mode;
+    if (m == 0) carry = halves(std::integral_constant<int, 0>{});
+    else if (m == 1) carry = halves(std::integral_constant<int, 1>{});
+    else carry = halves(std::integral_constant<int, 2>{});
+  }
 
   // ---- the tile's record for the fix-up: aggregate and open end
   if (threadIdx.x == 0) {

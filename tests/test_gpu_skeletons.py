@@ -380,19 +380,24 @@ def test_fold_seg_rows_unaligned(stream):
     assert np.array_equal(o.to_numpy(stream), O.fold_seg(ADD, np.int32(-1), x, off, outer=outer))
 
 
+@pytest.mark.parametrize("empties", [False, True])
 @pytest.mark.parametrize("maxlen", [3, 40, 300, 70000])
 @pytest.mark.parametrize("itype", [np.int32, np.int64])
-def test_fold_seg_starts_per_tile(stream, maxlen, itype):
+def test_fold_seg_starts_per_tile(stream, maxlen, itype, empties):
     # ~8000, ~400, ~50 and < 1 segment starts per 8192-element Int64 tile: the
-    # staged-in-HBM, staged-in-LDS, handed-over and carry-across-tiles paths
+    # staged-from-the-offsets, handed-over and carry-across-tiles paths; with
+    # distinct starts an end's segment is a count of heads, with empty
+    # segments (repeated starts) a search over the starts in LDS or, past 512
+    # starts in a tile, in the offsets
     total = 3 * (1 << 20) + 17
     nseg = max(2, total // max(1, maxlen // 2))
-    off = make_offsets(nseg, maxlen, total, 21, itype)
+    off = make_offsets(nseg, maxlen, total, 21, itype, empties=empties)
     n = int(off[-1])
     x = rand(np.int64, n, 22)
     d, do = dev(stream, x), dev(stream, off)
-    o = A.DeviceArray(nseg, np.int64)
     for op, seed in ((ADD, None), (MIN, np.int64(5)), (XOR, None)):
+        # without a seed (fold1Seg) empty segments keep what the output held
+        o = dev(stream, np.zeros(nseg, np.int64))
         A.fold_seg(stream, op, seed, d, 1, n, do, nseg, o)
         assert np.array_equal(o.to_numpy(stream), O.fold_seg(op, seed, x, off.astype(np.int64))), (op, maxlen)
 
@@ -547,6 +552,21 @@ def test_stencil_int_interior_tiles(stream, dt, r, shape):
     for b in (CLAMP, WRAP):
         A.stencil2d(stream, ry, rx, wt, b, None, dev(stream, img), h, w, out)
         assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(ry, rx, wt, b, None, img)), b
+
+
+@pytest.mark.parametrize("r", [(1, 1), (2, 2)])
+def test_stencil_int_large_image(stream, r):
+    # 2100 x 4096 Int32: 1922 interior tiles (XCD bands of 240-241 tiles) and a
+    # frame of 190 border tiles; every boundary, calls back to back on one stream
+    ry, rx = r
+    h, w = 2100, 4096
+    img = np.random.default_rng(h + ry).integers(-1000, 1000, size=(h, w)).astype(np.int32)
+    wt = np.random.default_rng(9).integers(-3, 4, size=(2 * ry + 1, 2 * rx + 1)).astype(np.int32)
+    d = dev(stream, img)
+    out = A.DeviceArray(h * w, np.int32)
+    for b in (CLAMP, MIRROR, WRAP, CONSTANT):
+        A.stencil2d(stream, ry, rx, wt, b, np.int32(7), d, h, w, out)
+        assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(ry, rx, wt, b, np.int32(7), img)), b
 
 
 def test_stencil_float64_special(stream):

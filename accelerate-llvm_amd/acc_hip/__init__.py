@@ -27,6 +27,12 @@ ADD, MUL, MIN, MAX, AND, OR, XOR, SUB, EXCHANGE = range(9)
 LEFT_TO_RIGHT, RIGHT_TO_LEFT = 0, 1
 # acc_boundary
 CLAMP, MIRROR, WRAP, CONSTANT = range(4)
+# acc_unop (map / transform element functions)
+UNOP_ID, UNOP_NEG, UNOP_ABS, UNOP_AFFINE, UNOP_CAST = range(5)
+# acc_genop
+GEN_FILL, GEN_IOTA = range(2)
+# acc_ixmap (transform / backpermute index maps)
+IX_GATHER, IX_REVERSE, IX_TRANSPOSE = range(3)
 
 _NP_TO_ACC = {
     np.dtype(np.int32): INT32,
@@ -177,6 +183,11 @@ SIGNATURES = {
                                 c_void_p, c_int64]),
     "acc_hip_stencil2d": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int64,
                                   c_int64, c_void_p]),
+    "acc_hip_map": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "acc_hip_zipwith": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "acc_hip_generate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
+    "acc_hip_transform": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_int,
+                                  c_void_p, c_int64, c_void_p]),
     "acc_hip_batch_create": (c_int, [c_int, POINTER(BatchProblem), c_int64, c_int, POINTER(c_void_p)]),
     "acc_hip_batch_launch": (c_int, [c_void_p, c_void_p]),
     "acc_hip_batch_tile": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_uint32)]),
@@ -395,6 +406,35 @@ def stencil2d(stream, ry, rx, weights, boundary, constant, src, height, width, o
     keep, cp = _scalar(dt, constant)
     check(lib().acc_hip_stencil2d(stream.handle, acc_type(dt), ry, rx, w.ctypes.data_as(c_void_p), boundary, cp,
                                   _p(src), height, width, _p(out)))
+
+
+def _pair(dtype, a, b=0):
+    v = np.array([a, b], dtype=dtype)
+    return v, v.ctypes.data_as(c_void_p)
+
+
+def map_(stream, f, src, n, out, a=None, b=None):
+    """out[i] = f(src[i]); f one of UNOP_*; UNOP_AFFINE takes a and b; UNOP_CAST converts
+    src's element type to out's."""
+    keep, pp = _pair(out.dtype, a, b) if f == UNOP_AFFINE else (None, None)
+    check(lib().acc_hip_map(stream.handle, acc_type(src.dtype), acc_type(out.dtype), f, pp, _p(src), n, _p(out)))
+
+
+def zipwith(stream, op, a, b, n, out):
+    check(lib().acc_hip_zipwith(stream.handle, acc_type(out.dtype), op, _p(a), _p(b), n, _p(out)))
+
+
+def generate(stream, g, n, out, a, b=0):
+    keep, pp = _pair(out.dtype, a, b)
+    check(lib().acc_hip_generate(stream.handle, acc_type(out.dtype), g, pp, n, _p(out)))
+
+
+def transform(stream, p, f, src, in_outer, in_inner, index, n, out, a=None, b=None):
+    """out[i] = f(src[p(i)]); p one of IX_*, f one of UNOP_ID/NEG/ABS/AFFINE (backpermute: UNOP_ID)."""
+    keep, pp = _pair(out.dtype, a, b) if f == UNOP_AFFINE else (None, None)
+    it = acc_type(index.dtype) if index is not None else INT32
+    check(lib().acc_hip_transform(stream.handle, acc_type(out.dtype), p, f, pp, _p(src), in_outer, in_inner, it,
+                                  _p(index), n, _p(out)))
 
 
 def _bits(dtype, v):

@@ -243,6 +243,65 @@ int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, const void* we
                       acc_boundary boundary, const void* constant, const void* in,
                       int64_t height, int64_t width, void* out);
 
+/* ================================================================ producers
+ * map, zipWith, generate, transform and backpermute (LLVM/Execute.hs:310-313;
+ * launched by mapOp, generateOp, transformOp and backpermuteOp,
+ * PTX/Execute.hs:117-206; kernels PTX/CodeGen/{Map,Generate,Transform}.hs).
+ * The reference compiles the program's own element and index functions into
+ * these kernels; here they come from the fixed sets below.  Integer
+ * arithmetic wraps; floating a*x + b and a + i*b round after each step (no
+ * fused multiply-add).  All arrays are device pointers, linear (an array of
+ * any shape is its row-major elements). */
+
+/* Element functions of map and transform. */
+typedef enum acc_unop {
+  ACC_UNOP_ID = 0,     /* x */
+  ACC_UNOP_NEG = 1,    /* negate x = x * (-1)             (LLVM/CodeGen/Arithmetic.hs:69-73) */
+  ACC_UNOP_ABS = 2,    /* fabs / llvm.abs, abs minBound == minBound, identity if unsigned (:75-86) */
+  ACC_UNOP_AFFINE = 3, /* a * x + b; params: two host elements {a, b} of the array's type */
+  ACC_UNOP_CAST = 4    /* C conversion of the input type to the output type (map only):
+                          integers wrap, floating to integer truncates (in-range values) */
+} acc_unop;
+
+/* Generators. */
+typedef enum acc_genop {
+  ACC_GEN_FILL = 0, /* out[i] = a;          params: {a} */
+  ACC_GEN_IOTA = 1  /* out[i] = a + i * b;  params: {a, b} (enumFromStepN) */
+} acc_genop;
+
+/* Index maps of transform / backpermute: out[i] = f(in[p(i)]). */
+typedef enum acc_ixmap {
+  ACC_IX_GATHER = 0,   /* p(i) = index[i], a linear index into in; an index outside
+                          [0, in_outer * in_inner) gives 0 and reads nothing */
+  ACC_IX_REVERSE = 1,  /* p(i) = n - 1 - i */
+  ACC_IX_TRANSPOSE = 2 /* in is in_outer x in_inner, out is in_inner x in_outer:
+                          out[c][r] = f(in[r][c]) */
+} acc_ixmap;
+
+/* map (PTX/Execute.hs:140-162, PTX/CodeGen/Map.hs:37-63): out[i] = f(in[i]),
+ * i < n.  t_in == t_out except for ACC_UNOP_CAST.  in == out maps in place
+ * (the reference's in-place case, PTX/Execute.hs:151-158). */
+int acc_hip_map(acc_stream_t s, acc_type t_in, acc_type t_out, acc_unop f, const void* params,
+                const void* in, int64_t n, void* out);
+
+/* zipWith f a b, which the frontend delivers as a generate reading both arrays
+ * (PTX/Execute.hs:164-173, PTX/CodeGen/Generate.hs:37-62): out[i] = f(a[i], b[i])
+ * for f in ADD MUL MIN MAX AND OR XOR (as in fold) and SUB (a[i] - b[i]). */
+int acc_hip_zipwith(acc_stream_t s, acc_type t, acc_op f, const void* a, const void* b, int64_t n,
+                    void* out);
+
+/* generate (PTX/Execute.hs:164-173, PTX/CodeGen/Generate.hs:37-62). */
+int acc_hip_generate(acc_stream_t s, acc_type t, acc_genop g, const void* params, int64_t n,
+                     void* out);
+
+/* transform and backpermute (PTX/Execute.hs:175-206, PTX/CodeGen/Transform.hs:37-67;
+ * backpermute is f = ACC_UNOP_ID).  f in {ID, NEG, ABS, AFFINE}.  n is the
+ * output's element count (= in_outer * in_inner for REVERSE and TRANSPOSE).
+ * index_type (ACC_INT32 / ACC_INT64) and index are used by ACC_IX_GATHER. */
+int acc_hip_transform(acc_stream_t s, acc_type t, acc_ixmap p, acc_unop f, const void* params,
+                      const void* in, int64_t in_outer, int64_t in_inner, acc_type index_type,
+                      const void* index, int64_t n, void* out);
+
 /* ================================================================ batched
  * Many independent scans and folds of uneven sizes in ONE persistent launch
  * (no counterpart in the reference, whose per-call path is

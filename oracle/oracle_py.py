@@ -128,6 +128,106 @@ def stencil2d(ry, rx, weights, boundary, constant, img):
     return out
 
 
+# ---- producers (numpy restatements; numpy float32/float64 arithmetic rounds
+# after every operation, integer arrays wrap, astype is the C conversion)
+UNOP_ID, UNOP_NEG, UNOP_ABS, UNOP_AFFINE, UNOP_CAST = range(5)
+GEN_FILL, GEN_IOTA = range(2)
+IX_GATHER, IX_REVERSE, IX_TRANSPOSE = range(3)
+
+
+def _wrap_int(fn, x, *args):
+    """fn over the unsigned type of the same width (wrapping), back to x's type."""
+    u = np.dtype(f"u{x.dtype.itemsize}")
+    return fn(x.view(u), *[np.asarray(a, dtype=x.dtype).astype(u) for a in args]).astype(u).view(x.dtype)
+
+
+def map_unop(f, x, out_dtype=None, a=None, b=None):
+    """map f (PTX/CodeGen/Map.hs:37-63) with the element functions of
+    LLVM/CodeGen/Arithmetic.hs:69-86: negate x = x * (-1), abs = fabs /
+    llvm.abs (abs minBound == minBound) / identity for unsigned types."""
+    x = np.ascontiguousarray(x)
+    if f == UNOP_CAST:
+        return x.astype(out_dtype)
+    if f == UNOP_ID:
+        return x.copy()
+    integral = x.dtype.kind in "iu"
+    if f == UNOP_NEG:
+        return _wrap_int(lambda v: np.uint64(0).astype(v.dtype) - v, x) if integral else -x
+    if f == UNOP_ABS:
+        if x.dtype.kind == "u":
+            return x.copy()
+        if integral:
+            return np.where(x < 0, _wrap_int(lambda v: np.uint64(0).astype(v.dtype) - v, x), x)
+        return np.abs(x)
+    if f == UNOP_AFFINE:
+        if integral:
+            return _wrap_int(lambda v, aa, bb: v * aa + bb, x, a, b)
+        t = x.dtype.type
+        return (t(a) * x) + t(b)
+    raise ValueError(f"element function {f}")
+
+
+def zipwith(op, a, b):
+    """zipWith f a b for the combine functions of fold plus SUB (a - b)."""
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    integral = a.dtype.kind in "iu"
+    if op in (0, 1, 7) and integral:  # ADD MUL SUB wrap
+        fn = {0: np.add, 1: np.multiply, 7: np.subtract}[op]
+        return _wrap_int(lambda u, v: fn(u, v), a, b)
+    if op == 0:
+        return a + b
+    if op == 1:
+        return a * b
+    if op == 7:
+        return a - b
+    if op == 2:
+        return np.fmin(a, b) if not integral else np.minimum(a, b)
+    if op == 3:
+        return np.fmax(a, b) if not integral else np.maximum(a, b)
+    if op == 4:
+        return a & b
+    if op == 5:
+        return a | b
+    if op == 6:
+        return a ^ b
+    raise ValueError(f"combine {op}")
+
+
+def generate(g, n, dtype, a, b=0):
+    """generate (PTX/CodeGen/Generate.hs:37-62) with fill a or iota a + i*b."""
+    dtype = np.dtype(dtype)
+    if g == GEN_FILL:
+        return np.full(n, a, dtype=dtype)
+    if g == GEN_IOTA:
+        if dtype.kind in "iu":
+            u = np.dtype(f"u{dtype.itemsize}")
+            au = np.asarray(a, dtype=dtype).astype(u)
+            bu = np.asarray(b, dtype=dtype).astype(u)
+            return (au + np.arange(n, dtype=np.uint64).astype(u) * bu).astype(u).view(dtype)
+        t = dtype.type
+        return t(a) + np.arange(n, dtype=np.int64).astype(dtype) * t(b)
+    raise ValueError(f"generator {g}")
+
+
+def transform(p, f, x, in_outer, in_inner, index=None, n=None, a=None, b=None):
+    """transform / backpermute (PTX/CodeGen/Transform.hs:37-67): out[i] =
+    f(x[p(i)]); a gather index outside x gives 0."""
+    x = np.ascontiguousarray(x).reshape(-1)
+    if p == IX_GATHER:
+        idx = np.asarray(index, dtype=np.int64)
+        ok = (idx >= 0) & (idx < x.size)
+        src = np.zeros(idx.size, x.dtype)
+        src[ok] = x[idx[ok]]
+        y = map_unop(f, src, a=a, b=b)
+        y[~ok] = 0
+        return y
+    if p == IX_REVERSE:
+        return map_unop(f, x[::-1].copy(), a=a, b=b)
+    if p == IX_TRANSPOSE:
+        return map_unop(f, x.reshape(in_outer, in_inner).T.copy().reshape(-1), a=a, b=b)
+    raise ValueError(f"index map {p}")
+
+
 # ---- FP64 truth for Float32 data
 def fold_f64(x, seed=None):
     x = np.ascontiguousarray(x, dtype=np.float32)

@@ -301,3 +301,81 @@ def test_gen_segment_lengths_dense_stay_positive():
     for nseg, maxlen, total in ((3145745, 3, 3145745), (1000, 5, 1200), (7, 1000, 7)):
         lens = O.gen_segment_lengths(21, nseg, maxlen, total)
         assert lens.min() >= 1 and int(lens.sum()) == total
+
+
+# ------------------------------------------------------------------ producers
+def test_iota_closed_form_and_rounding():
+    # Σ (a + i*b) for i < n = n*a + b*n(n-1)/2, taken mod 2^32 for Int32
+    n, a, b = 100003, -7, 3
+    y = O.generate(O.GEN_IOTA, n, np.int32, a, b)
+    want = (n * a + b * n * (n - 1) // 2) % 2**32
+    assert int(y.astype(np.int64).sum()) % 2**32 == want
+    assert y[0] == a and y[-1] == a + (n - 1) * b
+    # wrapping: 2^31 - 1 + 1 is minBound
+    assert O.generate(O.GEN_IOTA, 2, np.int32, 2**31 - 1, 1)[1] == -2**31
+    # Float32: i is converted (rounded) before the product: (float)(2^24 + 1) == 2^24
+    f = O.generate(O.GEN_IOTA, 2**24 + 2, np.float32, 0.0, 1.0)
+    assert f[2**24 + 1] == np.float32(2**24) and f[12345] == np.float32(12345)
+    assert np.array_equal(O.generate(O.GEN_IOTA, 4, np.float32, 0.5, 0.25), np.float32([0.5, 0.75, 1.0, 1.25]))
+    assert np.array_equal(O.generate(O.GEN_FILL, 3, np.float64, 2.5), [2.5, 2.5, 2.5])
+
+
+def test_negate_abs_known_answers():
+    # LLVM/CodeGen/Arithmetic.hs:69-86: wrapping negate; abs minBound == minBound
+    i32 = np.array([-2**31, -5, 0, 7, 2**31 - 1], np.int32)
+    assert np.array_equal(O.map_unop(O.UNOP_NEG, i32), np.array([-2**31, 5, 0, -7, -2**31 + 1], np.int32))
+    assert np.array_equal(O.map_unop(O.UNOP_ABS, i32), np.array([-2**31, 5, 0, 7, 2**31 - 1], np.int32))
+    u = np.array([0, 1, 2**64 - 1], np.uint64)
+    assert np.array_equal(O.map_unop(O.UNOP_ABS, u), u)
+    assert np.array_equal(O.map_unop(O.UNOP_NEG, u), np.array([0, 2**64 - 1, 1], np.uint64))
+    f = O.map_unop(O.UNOP_ABS, np.array([-0.0, -1.5, np.inf, -np.inf], np.float32))
+    assert np.array_equal(f, [0.0, 1.5, np.inf, np.inf]) and not np.signbit(f[0])
+    assert np.signbit(O.map_unop(O.UNOP_NEG, np.array([0.0], np.float64))[0])
+
+
+def test_affine_rounds_each_step():
+    # 3 * float32(1/3) rounds to 1.0, so (3x) - 1 is 0; a fused multiply-add
+    # would give 3 * 0.333333343... - 1 = 2.98e-8
+    x = np.array([np.float32(1.0) / np.float32(3.0)], np.float32)
+    assert O.map_unop(O.UNOP_AFFINE, x, a=3.0, b=-1.0)[0] == 0.0
+    # integers wrap: 2 * 2^30 is minBound
+    assert O.map_unop(O.UNOP_AFFINE, np.array([2**30], np.int32), a=2, b=0)[0] == -2**31
+    assert O.map_unop(O.UNOP_AFFINE, np.array([5], np.int64), a=-3, b=4)[0] == -11
+
+
+def test_conversion_known_answers():
+    # C conversions: truncation toward zero, wrapping narrowing, nearest-even rounding
+    assert np.array_equal(O.map_unop(O.UNOP_CAST, np.float32([2.7, -2.7, 0.5]), np.int32), [2, -2, 0])
+    assert O.map_unop(O.UNOP_CAST, np.array([2**32 + 5], np.int64), np.int32)[0] == 5
+    assert O.map_unop(O.UNOP_CAST, np.array([-1], np.int32), np.uint32)[0] == 2**32 - 1
+    assert O.map_unop(O.UNOP_CAST, np.array([2**53 + 1], np.int64), np.float64)[0] == 2.0**53
+    assert O.map_unop(O.UNOP_CAST, np.array([16777217], np.int32), np.float32)[0] == np.float32(16777216)
+    assert O.map_unop(O.UNOP_CAST, np.array([-1], np.int32), np.int64)[0] == -1
+
+
+def test_zipwith_known_answers():
+    a = np.array([-2**31, 5, 7], np.int32)
+    b = np.array([1, -3, 7], np.int32)
+    assert np.array_equal(O.zipwith(7, a, b), [2**31 - 1, 8, 0])  # SUB wraps
+    assert np.array_equal(O.zipwith(0, a, b), [-2**31 + 1, 2, 14])
+    assert np.array_equal(O.zipwith(1, np.array([2**16], np.int32), np.array([2**16], np.int32)), [0])
+    f = O.zipwith(3, np.float32([np.nan, 1.0]), np.float32([2.0, np.nan]))  # fmaxf: NaN loses
+    assert np.array_equal(f, [2.0, 1.0])
+    assert np.array_equal(O.zipwith(6, np.array([12], np.uint32), np.array([10], np.uint32)), [6])
+
+
+def test_transform_involutions_and_gather():
+    x = np.arange(60, dtype=np.int64) * 3 - 17
+    # reverse and transpose are involutions; transpose of a 1-row array is itself
+    r = O.transform(O.IX_REVERSE, O.UNOP_ID, x, 1, 60)
+    assert np.array_equal(O.transform(O.IX_REVERSE, O.UNOP_ID, r, 1, 60), x) and r[0] == x[-1]
+    t = O.transform(O.IX_TRANSPOSE, O.UNOP_ID, x, 6, 10)
+    assert np.array_equal(O.transform(O.IX_TRANSPOSE, O.UNOP_ID, t, 10, 6), x)
+    assert t[1] == x[10] and t[6] == x[1]
+    assert np.array_equal(O.transform(O.IX_TRANSPOSE, O.UNOP_ID, x, 1, 60), x)
+    # gather: np.take for in-range indices, 0 for the others; f applies after the gather
+    idx = np.array([5, 0, 59, -1, 60, 7], np.int32)
+    g = O.transform(O.IX_GATHER, O.UNOP_NEG, x, 1, 60, idx, 6)
+    assert np.array_equal(g[[0, 1, 2, 5]], -np.take(x, [5, 0, 59, 7])) and g[3] == 0 and g[4] == 0
+    perm = np.random.default_rng(1).permutation(60)
+    assert np.array_equal(O.transform(O.IX_GATHER, O.UNOP_ID, x, 1, 60, perm, 60), np.take(x, perm))

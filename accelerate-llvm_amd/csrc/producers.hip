@@ -230,11 +230,17 @@ __global__ __launch_bounds__(PB) void k_gen(T* out, int64_t n, G g, int aligned)
     return;
   }
   const int64_t nchunk = n / E;
-  for (int64_t c = (int64_t)blockIdx.x * PB + threadIdx.x; c < nchunk; c += stride) {
-    Vec<T> y;
+  for (int64_t c0 = (int64_t)blockIdx.x * PB + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
 #pragma unroll
-    for (int j = 0; j < E; ++j) y.e[j] = g(c * E + j);
-    store16_stream(out + c * E, y);
+    for (int u = 0; u < PU; ++u) {
+      const int64_t c = c0 + u * stride;
+      if (c < nchunk) {
+        Vec<T> y;
+#pragma unroll
+        for (int j = 0; j < E; ++j) y.e[j] = g(c * E + j);
+        store16_stream(out + c * E, y);
+      }
+    }
   }
   if (blockIdx.x == 0) {
     const int64_t i = nchunk * E + threadIdx.x;
@@ -478,7 +484,7 @@ extern "C" int acc_hip_generate(acc_stream_t s, acc_type t, acc_genop g, const v
     __builtin_memcpy(&a, params, sizeof(T));
     if (g == ACC_GEN_IOTA) __builtin_memcpy(&b, (const char*)params + sizeof(T), sizeof(T));
     const int aligned = al16(out) ? 1 : 0;
-    const unsigned grid = stream_grid(n / (16 / (int64_t)sizeof(T)) + 1, di->compute_units);
+    const unsigned grid = stream_grid(n / (16 / (int64_t)sizeof(T)) + 1, di->compute_units);  // PU chunks per thread
     if (g == ACC_GEN_FILL)
       hipLaunchKernelGGL((k_gen<T, GFill<T>>), dim3(grid), dim3(PB), 0, s->stream, (T*)out, n, GFill<T>{a}, aligned);
     else if (g == ACC_GEN_IOTA)

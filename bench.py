@@ -469,6 +469,22 @@ def main():
                timed(s, lambda: A.scan(s, A.ADD, A.LEFT_TO_RIGHT, None, dx, 1, n28, dy), R))
         record("scanl_add_i32_2^28", n28 * 4 + (n28 + 1) * 4,
                timed(s, lambda: A.scan(s, A.ADD, A.LEFT_TO_RIGHT, np.int32(0), dx, 1, n28 - 1, dy), R))
+        record("scanl1_rows_add_i32_2^18x1024", 2 * n28 * 4,
+               timed(s, lambda: A.scan(s, A.ADD, A.LEFT_TO_RIGHT, None, dx, 1 << 18, 1024, dy), R))
+        # producers
+        dz = A.DeviceArray(n28, np.int32)
+        record("generate_iota_i32_2^28", n28 * 4, timed(s, lambda: A.generate(s, A.GEN_IOTA, n28, dz, 1, 3), R))
+        record("map_affine_i32_2^28", 2 * n28 * 4, timed(s, lambda: A.map_(s, A.UNOP_AFFINE, dx, n28, dy, 3, -7), R))
+        record("zipWith_add_i32_2^28", 3 * n28 * 4, timed(s, lambda: A.zipwith(s, A.ADD, dx, dz, n28, dy), R))
+        record("reverse_i32_2^28", 2 * n28 * 4,
+               timed(s, lambda: A.transform(s, A.IX_REVERSE, A.UNOP_ID, dx, 1, n28, None, n28, dy), R))
+        record("transpose_i32_16384^2", 2 * n28 * 4,
+               timed(s, lambda: A.transform(s, A.IX_TRANSPOSE, A.UNOP_ID, dx, 1 << 14, 1 << 14, None, n28, dy), R))
+        k26 = 1 << 26
+        d_perm = A.DeviceArray.from_numpy(s, np.random.default_rng(SEED).permutation(k26).astype(np.int32))
+        record("backpermute_random_i32_2^26", 3 * k26 * 4,
+               timed(s, lambda: A.transform(s, A.IX_GATHER, A.UNOP_ID, dx, 1, k26, d_perm, k26, dy), R))
+        del dz, d_perm
         del dx, dy
         record("fold1Seg_max_f32_2^26", k_bytes,
                timed(s, lambda: A.fold_seg(s, A.MAX, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_max), R))

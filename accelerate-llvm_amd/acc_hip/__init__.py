@@ -183,6 +183,8 @@ SIGNATURES = {
                                 c_void_p, c_int64]),
     "acc_hip_stencil2d": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int64,
                                   c_int64, c_void_p]),
+    "acc_hip_stencil3d": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                                  c_int64, c_int64, c_int64, c_void_p]),
     "acc_hip_map": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "acc_hip_zipwith": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "acc_hip_generate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
@@ -406,6 +408,14 @@ def stencil2d(stream, ry, rx, weights, boundary, constant, src, height, width, o
     keep, cp = _scalar(dt, constant)
     check(lib().acc_hip_stencil2d(stream.handle, acc_type(dt), ry, rx, w.ctypes.data_as(c_void_p), boundary, cp,
                                   _p(src), height, width, _p(out)))
+
+
+def stencil3d(stream, rz, ry, rx, weights, boundary, constant, src, depth, height, width, out):
+    dt = src.dtype
+    w = np.ascontiguousarray(np.asarray(weights, dtype=dt).reshape(-1))
+    keep, cp = _scalar(dt, constant)
+    check(lib().acc_hip_stencil3d(stream.handle, acc_type(dt), rz, ry, rx, w.ctypes.data_as(c_void_p), boundary,
+                                  cp, _p(src), depth, height, width, _p(out)))
 
 
 def _pair(dtype, a, b=0):

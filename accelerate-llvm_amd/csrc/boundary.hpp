@@ -1,0 +1,27 @@
+// The stencil boundary index map, shared by the 2-D and 3-D stencils.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/accelerate_hip.h"
+
+namespace acc {
+
+// Coordinate i of an extent n mapped into [0, n) for clamp, mirror and wrap
+// (LLVM/CodeGen/Stencil.hs:224-253: one step of the map, as the reference
+// applies it); a constant boundary's coordinates are mapped too (to any
+// index inside: the caller substitutes the constant).
+__device__ __forceinline__ int64_t bound_index(int b, int64_t i, int64_t n) {
+  if (i < 0) {
+    if (b == ACC_BOUNDARY_CLAMP) i = 0;
+    else if (b == ACC_BOUNDARY_MIRROR) i = -i;
+    else if (b == ACC_BOUNDARY_WRAP) i = n + i;
+  } else if (i >= n) {
+    if (b == ACC_BOUNDARY_CLAMP) i = n - 1;
+    else if (b == ACC_BOUNDARY_MIRROR) i = n - (i - n + 2);
+    else if (b == ACC_BOUNDARY_WRAP) i = i - n;
+  }
+  // arrays smaller than the radius: keep the single-step map in bounds
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+}  // namespace acc

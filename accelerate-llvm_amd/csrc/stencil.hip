@@ -3,12 +3,13 @@
 // The reference launches one interior kernel with no bounds checks plus one
 // border kernel per face (2*rank launches on forked streams joined by
 // events, PTX/Execute.hs:686-776), and every tap is a separate global load
-// (PTX/CodeGen/Stencil.hs:113-177).  Here one launch covers interior and
-// border: each workgroup stages a (TH+2RY) x (TW+2RX) halo tile in LDS
-// (interior tiles with dwordx4 copies, border tiles applying the boundary map,
-// LLVM/CodeGen/Stencil.hs:224-253, to the coordinates outside the array as
-// they load), then each thread computes a 4x4 block of outputs from a register
-// window read from LDS with 16-byte reads and writes it with dwordx4 stores.
+// (PTX/CodeGen/Stencil.hs:113-177).  Here two launches on one stream cover
+// the interior tiles and the frame of border tiles: each workgroup stages a
+// (TH+2RY) x (TW+2RX) halo tile in LDS (interior tiles with dwordx4 copies,
+// border tiles applying the boundary map, LLVM/CodeGen/Stencil.hs:224-253, to
+// the coordinates outside the array as they load), then each thread computes
+// a 4x4 block of outputs from a register window read from LDS with 16-byte
+// reads and writes it with dwordx4 stores.
 //
 // The stencil function is f(tuple) = sum_{dy,dx} w[dy][dx] * x[y+dy][x+dx]
 // with taps in tuple order (rows outer, columns inner,
@@ -17,6 +18,7 @@
 //
 // HBM roofline: 2*H*W*sizeof(T) bytes (each input read once, each output
 // written once; halo re-reads are served by L2).
+#include "boundary.hpp"
 #include "dispatch.hpp"
 #include "vecmem.hpp"
 
@@ -41,19 +43,6 @@ template <typename T> struct StArgs {
   T wt[81];  // (2RY+1) x (2RX+1) weights, row-major
 };
 
-__device__ __forceinline__ int64_t bound_index(int b, int64_t i, int64_t n) {
-  if (i < 0) {
-    if (b == ACC_BOUNDARY_CLAMP) i = 0;
-    else if (b == ACC_BOUNDARY_MIRROR) i = -i;
-    else if (b == ACC_BOUNDARY_WRAP) i = n + i;
-  } else if (i >= n) {
-    if (b == ACC_BOUNDARY_CLAMP) i = n - 1;
-    else if (b == ACC_BOUNDARY_MIRROR) i = n - (i - n + 2);
-    else if (b == ACC_BOUNDARY_WRAP) i = i - n;
-  }
-  // arrays smaller than the radius: keep the single-step map in bounds
-  return i < 0 ? 0 : (i >= n ? n - 1 : i);
-}
 
 // Interior tiles (INTERIOR): the halo, rounded out to 16-byte boundaries,
 // lies inside the array and is copied with dwordx4 vectors.  Border tiles:

@@ -243,6 +243,16 @@ int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, const void* we
                       acc_boundary boundary, const void* constant, const void* in,
                       int64_t height, int64_t width, void* out);
 
+/* stencil1 on a DIM3 array (LLVM/Execute.hs:131-138; PTX/Execute.hs:652-776) with
+ * f = sum over taps of w[dz][dy][dx] * x[z+dz][y+dy][x+dx], taps in tuple order
+ * (planes, rows, columns), dz in [-rz, rz], dy in [-ry, ry], dx in [-rx, rx];
+ * `weights` is a host array of (2rz+1)*(2ry+1)*(2rx+1) elements of type t.  Radii
+ * 1..2 (3- and 5-point extents per axis); the array is depth x height x width,
+ * row-major.  Boundaries as for acc_hip_stencil2d, on every axis. */
+int acc_hip_stencil3d(acc_stream_t s, acc_type t, int rz, int ry, int rx, const void* weights,
+                      acc_boundary boundary, const void* constant, const void* in, int64_t depth,
+                      int64_t height, int64_t width, void* out);
+
 /* ================================================================ producers
  * map, zipWith, generate, transform and backpermute (LLVM/Execute.hs:310-313;
  * launched by mapOp, generateOp, transformOp and backpermuteOp,

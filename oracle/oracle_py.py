@@ -228,6 +228,55 @@ def transform(p, f, x, in_outer, in_inner, index=None, n=None, a=None, b=None):
     raise ValueError(f"index map {p}")
 
 
+# ---- 3-D stencil (numpy restatement)
+def _bound3(b, i, n):
+    """The boundary map of LLVM/CodeGen/Stencil.hs:224-253 (clamp 0, mirror 1,
+    wrap 2; constant 3 maps like clamp, the caller substitutes the value): one
+    step of the map, then clamped into range (extents below the radius)."""
+    i = np.asarray(i, dtype=np.int64)
+    if b == 1:
+        m = np.where(i < 0, -i, np.where(i >= n, 2 * n - 2 - i, i))
+    elif b == 2:
+        m = np.where(i < 0, i + n, np.where(i >= n, i - n, i))
+    else:
+        m = i
+    return np.clip(m, 0, n - 1)
+
+
+def _stencil3d(rz, ry, rx, weights, boundary, constant, vol, dtype):
+    vol = np.asarray(vol)
+    D, H, W = vol.shape
+    zs, ys, xs = np.arange(-rz, D + rz), np.arange(-ry, H + ry), np.arange(-rx, W + rx)
+    P = vol[np.ix_(_bound3(boundary, zs, D), _bound3(boundary, ys, H), _bound3(boundary, xs, W))].astype(dtype)
+    if boundary == 3:
+        out = ((zs < 0) | (zs >= D))[:, None, None] | ((ys < 0) | (ys >= H))[None, :, None] | \
+              ((xs < 0) | (xs >= W))[None, None, :]
+        P[out] = constant
+    w = np.asarray(weights, dtype=dtype).reshape(2 * rz + 1, 2 * ry + 1, 2 * rx + 1)
+    acc = None
+    for dz in range(2 * rz + 1):
+        for dy in range(2 * ry + 1):
+            for dx in range(2 * rx + 1):
+                t = w[dz, dy, dx] * P[dz:dz + D, dy:dy + H, dx:dx + W]
+                acc = t if acc is None else acc + t
+    return acc
+
+
+def stencil3d(rz, ry, rx, weights, boundary, constant, vol):
+    """stencil1 on a DIM3 array with the weighted-sum function, taps in tuple
+    order, in the element type (integers wrap)."""
+    vol = np.asarray(vol)
+    return _stencil3d(rz, ry, rx, weights, boundary, constant, vol, vol.dtype)
+
+
+def stencil3d_f64(rz, ry, rx, weights, boundary, constant, vol):
+    """FP64 truth and the sum of |w * x| per output (for the rounding bound)."""
+    t = _stencil3d(rz, ry, rx, weights, boundary, constant, vol, np.float64)
+    ab = _stencil3d(rz, ry, rx, np.abs(np.asarray(weights, np.float64)), boundary,
+                    abs(constant) if constant is not None else None, np.abs(np.asarray(vol, np.float64)), np.float64)
+    return t, ab
+
+
 # ---- FP64 truth for Float32 data
 def fold_f64(x, seed=None):
     x = np.ascontiguousarray(x, dtype=np.float32)

@@ -583,6 +583,57 @@ def test_stencil_float64_special(stream):
     assert np.array_equal(out.to_numpy(stream).reshape(9, 9)[2:7, 2:7], wt[::-1, ::-1])
 
 
+# ------------------------------------------------------------------ 3-D stencil
+S3_SHAPES = [(1, 1, 1), (2, 3, 4), (5, 17, 70), (33, 40, 130), (70, 20, 64)]
+
+
+@pytest.mark.parametrize("shape", S3_SHAPES)
+@pytest.mark.parametrize("r", [(1, 1, 1), (2, 2, 2), (1, 2, 1), (2, 1, 2)])
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+def test_stencil3d_int_exact(stream, dt, r, shape):
+    rz, ry, rx = r
+    d, h, w = shape
+    vol = np.random.default_rng(d * h + w).integers(-1000, 1000, size=shape).astype(dt)
+    wt = np.random.default_rng(rz * 9 + ry * 3 + rx).integers(-3, 4, size=(2 * rz + 1, 2 * ry + 1, 2 * rx + 1))
+    wt = wt.astype(dt)
+    src = dev(stream, vol)
+    out = A.DeviceArray(vol.size, dt)
+    for b in (CLAMP, MIRROR, WRAP, CONSTANT):
+        A.stencil3d(stream, rz, ry, rx, wt, b, dt(5), src, d, h, w, out)
+        got = out.to_numpy(stream).reshape(shape)
+        assert np.array_equal(got, O.stencil3d(rz, ry, rx, wt, b, dt(5), vol)), (b, shape)
+
+
+@pytest.mark.parametrize("shape", [(9, 33, 65), (64, 64, 64)])
+@pytest.mark.parametrize("r", [(1, 1, 1), (2, 2, 2)])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_stencil3d_float_bound(stream, dt, r, shape):
+    rz, ry, rx = r
+    d, h, w = shape
+    vol = O.gen_normal_f32(d + h + w, d * h * w).reshape(shape).astype(dt)
+    wt = np.random.default_rng(3).standard_normal((2 * rz + 1, 2 * ry + 1, 2 * rx + 1)).astype(dt)
+    out = A.DeviceArray(vol.size, dt)
+    eps = 2.0 ** -23 if dt == np.float32 else 2.0 ** -52
+    taps = wt.size
+    for b in (CLAMP, MIRROR, WRAP, CONSTANT):
+        A.stencil3d(stream, rz, ry, rx, wt, b, 0.25, dev(stream, vol), d, h, w, out)
+        got = out.to_numpy(stream).reshape(shape).astype(np.float64)
+        truth, ab = O.stencil3d_f64(rz, ry, rx, wt, b, 0.25, vol)
+        assert np.all(np.abs(got - truth) <= taps * eps * ab + 1e-30), b
+
+
+def test_stencil3d_large_volume(stream):
+    # 256^3 Float32, 3x3x3 clamp: several z runs per column of blocks
+    n = 256
+    vol = O.gen_normal_f32(9, n ** 3).reshape(n, n, n)
+    wt = np.full((3, 3, 3), 1.0 / 27.0, np.float32)
+    out = A.DeviceArray(vol.size, np.float32)
+    A.stencil3d(stream, 1, 1, 1, wt, CLAMP, None, dev(stream, vol), n, n, n, out)
+    got = out.to_numpy(stream).reshape(n, n, n).astype(np.float64)
+    truth, ab = O.stencil3d_f64(1, 1, 1, wt, CLAMP, None, vol)
+    assert np.all(np.abs(got - truth) <= 27 * 2.0 ** -23 * ab + 1e-30)
+
+
 # ------------------------------------------------------------------- runtime
 @pytest.mark.parametrize("nbytes", [16, 4096, (1 << 20) + 16, (1 << 24) + 48, 17, 1000003])
 @pytest.mark.parametrize("offset", [0, 4])

@@ -379,3 +379,37 @@ def test_transform_involutions_and_gather():
     assert np.array_equal(g[[0, 1, 2, 5]], -np.take(x, [5, 0, 59, 7])) and g[3] == 0 and g[4] == 0
     perm = np.random.default_rng(1).permutation(60)
     assert np.array_equal(O.transform(O.IX_GATHER, O.UNOP_ID, x, 1, 60, perm, 60), np.take(x, perm))
+
+
+# ----------------------------------------------------------------- 3-D stencil
+@pytest.mark.parametrize("b,mode", [(0, "nearest"), (1, "mirror"), (2, "wrap"), (3, "constant")])
+def test_stencil3d_matches_scipy(b, mode):
+    from scipy import ndimage
+    rng = np.random.default_rng(5)
+    vol = rng.standard_normal((7, 9, 11))
+    w = rng.standard_normal((3, 5, 3))
+    got = O.stencil3d(1, 2, 1, w, b, 0.5, vol)
+    want = ndimage.correlate(vol, w, mode=mode, cval=0.5)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_stencil3d_central_plane_is_the_2d_stencil():
+    # weights only in the central plane: each output plane is the (already
+    # pinned) 2-D stencil of its input plane; integers exact
+    rng = np.random.default_rng(6)
+    vol = rng.integers(-1000, 1000, size=(5, 13, 17)).astype(np.int32)
+    w2 = rng.integers(-3, 4, size=(3, 5)).astype(np.int32)
+    w3 = np.zeros((5, 3, 5), np.int32)
+    w3[2] = w2
+    for b in (0, 1, 2, 3):
+        got = O.stencil3d(2, 1, 2, w3, b, 7, vol)
+        want = np.stack([O.stencil2d(1, 2, w2, b, np.int32(7), vol[z]) for z in range(5)])
+        assert np.array_equal(got, want), b
+
+
+def test_stencil3d_impulse_gives_mirrored_weights():
+    vol = np.zeros((9, 9, 9))
+    vol[4, 4, 4] = 1.0
+    w = np.arange(27, dtype=np.float64).reshape(3, 3, 3)
+    got = O.stencil3d(1, 1, 1, w, 3, 0.0, vol)
+    assert np.array_equal(got[3:6, 3:6, 3:6], w[::-1, ::-1, ::-1])

@@ -507,6 +507,15 @@ def main():
                timed(s, lambda: A.stencil2d(s, 2, 2, w5, A.CLAMP, None, d_img, H, WD, d_out), R))
         record("stencil3x3_mirror_f32_8192^2", 2 * H * WD * 4,
                timed(s, lambda: A.stencil2d(s, 1, 1, w3, A.MIRROR, None, d_img, H, WD, d_out), R))
+        # 3-D: 3x3x3 box, clamp, on 512^3 Float32 (2^27 elements)
+        n3 = 512
+        d_vol = A.DeviceArray(n3 ** 3, np.float32)
+        d_vout = A.DeviceArray(n3 ** 3, np.float32)
+        A.generate(s, A.GEN_IOTA, n3 ** 3, d_vol, 0.0, 1e-6)
+        w27 = np.full((3, 3, 3), 1.0 / 27.0, np.float32)
+        record("stencil3d_3x3x3_clamp_f32_512^3", 2 * n3 ** 3 * 4,
+               timed(s, lambda: A.stencil3d(s, 1, 1, 1, w27, A.CLAMP, None, d_vol, n3, n3, n3, d_vout), R))
+        del d_vol, d_vout
         del d_out
 
     # ------------------------------------------------------- batched (C4)

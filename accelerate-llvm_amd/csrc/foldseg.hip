@@ -11,10 +11,11 @@
 // element of each segment is its result.
 //
 // Three launches, none of which waits on another workgroup:
-//   k_tile_seg  (offsets only) per tile: its first segment and the positions
-//               of up to SLOT segment starts inside it, so a tile stages its
-//               segment heads with one load issued ahead of its data; also
-//               writes the seed into empty segments;
+//   k_tile_seg  (offsets only) per tile boundary: the first segment of the
+//               tile starting there and the positions of its first SLOT
+//               segment starts, so a tile stages its segment heads with one
+//               load issued ahead of its data; also writes the seed into
+//               empty segments;
 //   k_fold_seg  one workgroup per tile: every segment end whose segment
 //               starts inside the tile is final at once and written; the one
 //               end that continues a segment from earlier tiles ("open") and
@@ -51,13 +52,11 @@ namespace acc {
 
 constexpr int SLOT = 64;  // segment starts per tile handed over by k_tile_seg
 
-// Per-tile segment range from k_tile_seg: starts g_lo .. g_lo + cnt - 1 cover
-// the tile (g_lo: the last segment starting at or before its first position,
-// 0 if none; the last one starts after its end or is past the last segment).
-struct TileSegs {
-  int64_t g_lo;
-  int64_t cnt;
-};
+// k_tile_seg's result per tile boundary b = 0 .. ntiles: first[b] is the
+// last segment starting at or before position b * TILE (0 if none).  Tile t
+// is covered by the starts first[t] .. first[t+1] + 1 (cnt = first[t+1] -
+// first[t] + 2 of them; the last starts after the tile or is past the last
+// segment).
 
 // Per-tile record for the fix-up.
 template <typename T> struct TileCarry {
@@ -87,7 +86,7 @@ template <typename T> struct CarryArrays {
 // Segment starts given by an offsets array: start(g) = off[g] + shift, g in [0, S].
 template <typename I> struct OffsetStarts {
   const I* off;
-  const TileSegs* segs;
+  const int64_t* first;  // per tile boundary (ntiles + 1)
   const int32_t* slots;  // SLOT tile-relative starts per tile (valid when cnt <= SLOT)
   int64_t shift;
   __device__ __forceinline__ int64_t start(int64_t g) const { return (int64_t)off[g] + shift; }
@@ -120,45 +119,37 @@ __device__ __forceinline__ int32_t rel_pos(int64_t p, int64_t base, int64_t tile
   return r < 0 ? -1 : (r > tile ? (int32_t)tile + 1 : (int32_t)r);
 }
 
-// One wave per tile: the tile's segment range by two 64-ary searches (one
-// round trip narrows both ranges 64-fold) and, when they fit, the tile-relative
-// positions of its starts.  Then, when a seed is given, every empty segment
-// gets the seed (grid-stride over the segments).
+// One wave per tile boundary: the boundary's first segment by one 64-ary
+// search (three round trips for 2^16 segments), and the positions of the
+// next tile's first SLOT starts relative to it.  (A search per boundary,
+// not two per tile: each probe is its own cache line, so the pre-pass is
+// bound by the probes' address processing, and the end of tile t is the
+// start of tile t+1.)  Then, when a seed is given, every empty segment gets
+// the seed (grid-stride over the segments).
 template <typename T, typename I>
 __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, int64_t tile, uint32_t ntiles,
-                                                  int64_t shift, TileSegs* segs, int32_t* slots, T* out,
+                                                  int64_t shift, int64_t* first, int32_t* slots, T* out,
                                                   uint32_t fill_empty, T seed) {
-  const uint32_t t = blockIdx.x * 4u + threadIdx.x / 64u;
+  const uint32_t t = blockIdx.x * 4u + threadIdx.x / 64u;  // boundary
   const int lane = lane_id();
-  if (t < ntiles) {  // whole waves
+  if (t <= ntiles) {  // whole waves
     const int64_t base = (int64_t)t * tile;  // position
-    // g_lo, g_hi: last g in [0, nseg] with off[g] <= e for e = the tile's
-    // first and one-past-last row element (0 if none); the two searches
-    // share their round trips
-    const int64_t ea = base - shift, eb = base + tile - shift;
-    const int64_t off0 = (int64_t)off[0];
-    int64_t la = 0, ha = off0 > ea ? 0 : nseg, lb = 0, hb = off0 > eb ? 0 : nseg;
-    while (ha > la || hb > lb) {
-      const int64_t sa = (ha - la + 64) / 64, sb = (hb - lb + 64) / 64;
-      const int64_t ga = la + (int64_t)lane * sa, gb = lb + (int64_t)lane * sb;
-      const bool pa = ha > la && ga <= ha, pb = hb > lb && gb <= hb;
-      const int64_t va = pa ? (int64_t)off[ga] : 0, vb = pb ? (int64_t)off[gb] : 0;
-      if (ha > la) {
-        const int c = __popcll(__ballot(pa && va <= ea));  // >= 1: off[la] <= ea
-        la = la + (int64_t)(c - 1) * sa;
-        ha = min(ha, la + sa - 1);
-      }
-      if (hb > lb) {
-        const int c = __popcll(__ballot(pb && vb <= eb));
-        lb = lb + (int64_t)(c - 1) * sb;
-        hb = min(hb, lb + sb - 1);
-      }
+    // last g in [0, nseg] with off[g] <= e for e = the row element at the
+    // boundary (0 if none)
+    const int64_t e = base - shift;
+    int64_t lo = 0, hi = (int64_t)off[0] > e ? 0 : nseg;
+    while (hi > lo) {
+      const int64_t st = (hi - lo + 64) / 64;
+      const int64_t g = lo + (int64_t)lane * st;
+      const bool p = g <= hi;
+      const int64_t v = p ? (int64_t)off[g] : 0;
+      const int c = __popcll(__ballot(p && v <= e));  // >= 1: off[lo] <= e
+      lo = lo + (int64_t)(c - 1) * st;
+      hi = min(hi, lo + st - 1);
     }
-    const int64_t g_lo = la, g_hi = lb;
-    const int64_t cnt = g_hi - g_lo + 2;
-    if (lane == 0) segs[t] = TileSegs{g_lo, cnt};
-    if (cnt <= SLOT && lane < cnt) {
-      const int64_t g = g_lo + lane;
+    if (lane == 0) first[t] = lo;
+    if (t < ntiles) {
+      const int64_t g = lo + lane;
       slots[(size_t)t * SLOT + lane] = g > nseg ? (int32_t)tile + 1 : rel_pos((int64_t)off[g] + shift, base, tile);
     }
   }
@@ -221,10 +212,11 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   // last element (an aligned 16-byte block with a valid element never
   // crosses a page); the run below turns padding into the identity.  The
   // staging overlaps the loads (barriers wait only for LDS traffic).
-  TileSegs hdr{0, 0};
+  int64_t f0 = 0, f1 = 0;
   int32_t my_rel = 0;
   if constexpr (!UNIFORM) {
-    hdr = a.st.segs[t];
+    f0 = a.st.first[t];
+    f1 = a.st.first[t + 1];
     my_rel = a.st.slots[(size_t)t * SLOT + lane];
   }
   T v[K][VEC];
@@ -255,8 +247,8 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
     }
     __syncthreads();
   } else {
-    g_lo = hdr.g_lo;
-    cnt = hdr.cnt;  // >= 2
+    g_lo = f0;
+    cnt = f1 - f0 + 2;  // >= 2
     if (cnt <= SLOT) {
       if (w == 0) {
         for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
@@ -574,7 +566,7 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
   const size_t cv_bytes = al256((size_t)ntiles * sizeof(T)), cf_bytes = al256((size_t)ntiles);
   const size_t cg_bytes = (size_t)ntiles * 8;
   const size_t carry_bytes = al256(2 * cv_bytes + cf_bytes + cg_bytes);
-  const size_t segs_bytes = offsets ? ((size_t)ntiles * sizeof(TileSegs) + 255) / 256 * 256 : 0;
+  const size_t segs_bytes = offsets ? ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256 : 0;
   const size_t slot_bytes = offsets ? (size_t)ntiles * SLOT * sizeof(int32_t) : 0;
   constexpr int FIX = 256;  // tiles per k_seg_fix workgroup
   const int64_t nfix = ceil_div(ntiles, (int64_t)FIX);
@@ -586,18 +578,18 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
                              (int64_t*)(cb + 2 * cv_bytes + cf_bytes)};
   const bool fill_empty = seed && check_empty;
   if constexpr (!std::is_same<Starts, UniformStarts>::value) {
-    TileSegs* segs = (TileSegs*)((char*)ws + carry_bytes);
+    int64_t* first = (int64_t*)((char*)ws + carry_bytes);
     int32_t* slots = (int32_t*)((char*)ws + carry_bytes + segs_bytes);
-    const unsigned pg = (unsigned)((ntiles + 3) / 4);
+    const unsigned pg = (unsigned)((ntiles + 1 + 3) / 4);  // a wave per tile boundary
     const T sd = seed ? *seed : T();
     if (offset_bytes == 4)
       hipLaunchKernelGGL((k_tile_seg<T, int32_t>), dim3(pg), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
-                         TILE, (uint32_t)ntiles, shift, segs, slots, out, fill_empty ? 1u : 0u, sd);
+                         TILE, (uint32_t)ntiles, shift, first, slots, out, fill_empty ? 1u : 0u, sd);
     else
       hipLaunchKernelGGL((k_tile_seg<T, int64_t>), dim3(pg), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
-                         TILE, (uint32_t)ntiles, shift, segs, slots, out, fill_empty ? 1u : 0u, sd);
+                         TILE, (uint32_t)ntiles, shift, first, slots, out, fill_empty ? 1u : 0u, sd);
     ACC_HIP_CHECK(hipGetLastError());
-    st.segs = segs;
+    st.first = first;
     st.slots = slots;
   }
   SegArgs<T, Starts> a{in0, out, st, nseg, shift, e_end + shift, (uint32_t)ntiles, seed ? 1u : 0u,

@@ -3,8 +3,8 @@
 // The reference launches one interior kernel with no bounds checks plus one
 // border kernel per face (2*rank launches on forked streams joined by
 // events, PTX/Execute.hs:686-776), and every tap is a separate global load
-// (PTX/CodeGen/Stencil.hs:113-177).  Here two launches on one stream cover
-// the interior tiles and the frame of border tiles: each workgroup stages a
+// (PTX/CodeGen/Stencil.hs:113-177).  Here one launch covers the interior
+// tiles and the frame of border tiles: each workgroup stages a
 // (TH+2RY) x (TW+2RX) halo tile in LDS (interior tiles with dwordx4 copies,
 // border tiles applying the boundary map, LLVM/CodeGen/Stencil.hs:224-253, to
 // the coordinates outside the array as they load), then each thread computes
@@ -43,14 +43,14 @@ template <typename T> struct StArgs {
   T wt[81];  // (2RY+1) x (2RX+1) weights, row-major
 };
 
-
-// Interior tiles (INTERIOR): the halo, rounded out to 16-byte boundaries,
-// lies inside the array and is copied with dwordx4 vectors.  Border tiles:
-// coordinates outside the array go through the boundary map (constant: the
-// value itself), element by element.  The two are separate launches: one
-// kernel branching between both staging loops has been compiled for gfx950
-// with the register holding the thread index left undefined on one path.
-template <typename T, int RY, int RX, bool INTERIOR>
+// Interior tiles: the halo, rounded out to 16-byte boundaries, lies inside
+// the array and is copied with dwordx4 vectors.  Border tiles: coordinates
+// outside the array go through the boundary map (constant: the value
+// itself), element by element.  MODE 2 (the one launched) holds both, blocks
+// [0, ni) taking the interior tiles; each staging path is straight-line code
+// (an earlier form that looped over both in one kernel was compiled with the
+// register holding the thread index undefined on one path).
+template <typename T, int RY, int RX, int MODE>  // 0 frame, 1 interior, 2 both
 __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   constexpr int VEC = Vec<T>::N;
   constexpr int PADL = (RX + VEC - 1) / VEC * VEC;  // halo rounded out to 16-byte boundaries
@@ -70,7 +70,9 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   // halo (a column-major walk fetched every such line twice from HBM); the
   // vertical halo is 2*RY of 64 rows.
   uint32_t tx, ty;
-  if constexpr (INTERIOR) {
+  const uint32_t nint = (a.ix1 - a.ix0) * (a.iy1 - a.iy0);  // interior tiles
+  const bool interior = MODE == 1 || (MODE == 2 && blockIdx.x < nint);
+  if (interior) {
     // workgroup b runs on XCD b % 8; XCD k takes tiles [start(k), start(k+1))
     // with nt / 8 tiles each and one more for the first nt % 8 XCDs
     const uint32_t nx = a.ix1 - a.ix0, nt = nx * (a.iy1 - a.iy0);
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   } else {
     // the frame around the interior rectangle: rows above it, rows below
     // it, then the left and right parts of the rows beside it
-    uint32_t b = blockIdx.x;
+    uint32_t b = MODE == 2 ? blockIdx.x - nint : blockIdx.x;
     const uint32_t top = a.iy0 * a.tiles_x, bottom = (a.tiles_y - a.iy1) * a.tiles_x;
     if (b < top) {
       ty = b / a.tiles_x;
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
   // ---- stage the halo tile.  Every load of the tile is issued before the
   // first LDS write (indexes past the tile re-read its last element, so the
   // loads need no branch and each wait names only the loads it needs).
-  if constexpr (INTERIOR) {
+  if (interior) {
     constexpr int NV = LS / VEC;  // vectors per LDS row
     constexpr int TOT = LH * NV;
     constexpr int NI = (TOT + ST_BLOCK - 1) / ST_BLOCK;
@@ -237,9 +239,8 @@ int stencil_launch(acc_stream_t s, const T* w, int boundary, const T* cval, cons
   for (int i = 0; i < 81; ++i) a.wt[i] = i < taps ? w[i] : T();
   const int64_t nt = tiles_x * tiles_y;
   const int64_t ni = (ix1 - ix0) * (iy1 - iy0);
-  if (ni > 0) hipLaunchKernelGGL((k_stencil2d<T, RY, RX, true>), dim3((unsigned)ni), dim3(ST_BLOCK), 0, s->stream, a);
-  if (nt > ni)
-    hipLaunchKernelGGL((k_stencil2d<T, RY, RX, false>), dim3((unsigned)(nt - ni)), dim3(ST_BLOCK), 0, s->stream, a);
+  // one launch: blocks [0, ni) take the interior tiles, the rest the frame
+  hipLaunchKernelGGL((k_stencil2d<T, RY, RX, 2>), dim3((unsigned)nt), dim3(ST_BLOCK), 0, s->stream, a);
   ACC_HIP_CHECK(hipGetLastError());
   return ACC_SUCCESS;
 }

@@ -63,7 +63,11 @@ __global__ __launch_bounds__(S3_BLOCK) void k_stencil3d(St3Args<T> a) {
   constexpr int KW = 2 * RX + 1, KH = 2 * RY + 1;
   __shared__ T ring[NP][PLANE];
 
-  const uint32_t b = blockIdx.x;
+  // XCD-aware block order: workgroup g runs on XCD g % 8, and XCD k takes the
+  // contiguous band of blocks [start(k), start(k+1)) (x fastest, then y, then
+  // the z run), so blocks that share halo rows share one L2
+  const uint32_t nb = gridDim.x, q = nb / 8u, rr = nb % 8u, xcd = blockIdx.x % 8u;
+  const uint32_t b = xcd * q + min(xcd, rr) + blockIdx.x / 8u;
   const uint32_t txb = b % a.tiles_x, tyb = (b / a.tiles_x) % a.tiles_y, zr = b / (a.tiles_x * a.tiles_y);
   // plain locals (no references to the kernel argument) and 32-bit row and
   // column coordinates (the host checks h, w < 2^31)

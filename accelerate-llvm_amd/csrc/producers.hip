@@ -153,13 +153,13 @@ __global__ __launch_bounds__(PB) void k_map(const Tin* in, Tout* out, int64_t n,
   constexpr int BI = E * (int)sizeof(Tin), BO = E * (int)sizeof(Tout);
   const int64_t nchunk = n / E;
   const int64_t stride = (int64_t)gridDim.x * PB;
-  for (int64_t c0 = (int64_t)blockIdx.x * PB + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
+  for (int64_t c0 = (int64_t)blockIdx.x * PB * PU + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
     Raw<BI> r[PU];
 #pragma unroll
-    for (int u = 0; u < PU; ++u) r[u] = ld_stream<BI>(in + min(c0 + u * stride, nchunk - 1) * E);
+    for (int u = 0; u < PU; ++u) r[u] = ld_stream<BI>(in + min(c0 + u * PB, nchunk - 1) * E);
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = c0 + u * stride;
+      const int64_t c = c0 + u * PB;
       if (c < nchunk) {
         Tin x[E];
         __builtin_memcpy(x, &r[u], BI);
@@ -190,17 +190,17 @@ __global__ __launch_bounds__(PB) void k_zip(const T* a, const T* b, T* out, int6
   constexpr int E = 16 / (int)sizeof(T);
   const int64_t nchunk = n / E;
   const int64_t stride = (int64_t)gridDim.x * PB;
-  for (int64_t c0 = (int64_t)blockIdx.x * PB + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
+  for (int64_t c0 = (int64_t)blockIdx.x * PB * PU + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
     Vec<T> va[PU], vb[PU];
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = min(c0 + u * stride, nchunk - 1);
+      const int64_t c = min(c0 + u * PB, nchunk - 1);
       va[u] = load16_stream(a + c * E);
       vb[u] = load16_stream(b + c * E);
     }
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = c0 + u * stride;
+      const int64_t c = c0 + u * PB;
       if (c < nchunk) {
         Vec<T> y;
 #pragma unroll
@@ -230,10 +230,10 @@ __global__ __launch_bounds__(PB) void k_gen(T* out, int64_t n, G g, int aligned)
     return;
   }
   const int64_t nchunk = n / E;
-  for (int64_t c0 = (int64_t)blockIdx.x * PB + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
+  for (int64_t c0 = (int64_t)blockIdx.x * PB * PU + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = c0 + u * stride;
+      const int64_t c = c0 + u * PB;
       if (c < nchunk) {
         Vec<T> y;
 #pragma unroll
@@ -258,10 +258,10 @@ __global__ __launch_bounds__(PB) void k_gather(const T* in, int64_t n_in, const 
   constexpr int BO = E * (int)sizeof(T);
   const int64_t nchunk = n / E;
   const int64_t stride = (int64_t)gridDim.x * PB;
-  for (int64_t c0 = (int64_t)blockIdx.x * PB + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
+  for (int64_t c0 = (int64_t)blockIdx.x * PB * PU + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
     Vec<I> iv[PU];
 #pragma unroll
-    for (int u = 0; u < PU; ++u) iv[u] = load16_stream(idx + min(c0 + u * stride, nchunk - 1) * E);
+    for (int u = 0; u < PU; ++u) iv[u] = load16_stream(idx + min(c0 + u * PB, nchunk - 1) * E);
     T x[PU][E];
 #pragma unroll
     for (int u = 0; u < PU; ++u)
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(PB) void k_gather(const T* in, int64_t n_in, const 
       }
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = c0 + u * stride;
+      const int64_t c = c0 + u * PB;
       if (c < nchunk) {
         T y[E];
 #pragma unroll
@@ -313,17 +313,17 @@ __global__ __launch_bounds__(PB) void k_reverse(const T* in, T* out, int64_t n, 
     return;
   }
   const int64_t nchunk = n / E;
-  for (int64_t c0 = (int64_t)blockIdx.x * PB + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
+  for (int64_t c0 = (int64_t)blockIdx.x * PB * PU + threadIdx.x; c0 < nchunk; c0 += stride * PU) {
     T x[PU][E];
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = min(c0 + u * stride, nchunk - 1);
+      const int64_t c = min(c0 + u * PB, nchunk - 1);
 #pragma unroll
       for (int j = 0; j < E; ++j) x[u][j] = in[n - 1 - (c * E + j)];
     }
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
-      const int64_t c = c0 + u * stride;
+      const int64_t c = c0 + u * PB;
       if (c < nchunk) {
         Vec<T> y;
 #pragma unroll
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(PB) void k_transpose(const T* in, T* out, int64_t r
 // ------------------------------------------------------------------- launch
 inline unsigned stream_grid(int64_t work, int cus) {
   const int64_t want = ceil_div(work, (int64_t)PB * PU);
-  const int64_t cap = (int64_t)cus * 8;
+  const int64_t cap = (int64_t)cus * 32;
   return (unsigned)max((int64_t)1, min(want, cap));
 }
 inline bool al16(const void* p) { return (uintptr_t)p % 16 == 0; }

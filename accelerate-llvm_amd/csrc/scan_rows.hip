@@ -39,6 +39,7 @@ template <typename T> struct RowScanArgs {
   T* seed_out;   // row r's seed at seed_out[r * seed_stride] (nullable)
   int64_t seed_stride;
   uint32_t ntiles;
+  uint32_t step_r;  // (64 * VEC) % inner: column step to one lane's next vector
   SyncCtl* ctl;
   uint64_t* status;
 };
@@ -87,13 +88,6 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
 
     // ---- load; per vector: traversal-ordered values, head bits (row starts
     // and padding), row and column of the first physical element
-    // row (from brow) and column of the first physical element of vector k
-    auto vec_pos = [&](int k, uint32_t* col, int64_t* row) {
-      const uint32_t cc = bcol + (uint32_t)(vidx(k) * VEC);
-      const uint32_t dr = cc / inner32;
-      *col = cc - dr * inner32;
-      *row = brow + dr;
-    };
     // column and row of physical slot j of a vector starting at (col, row):
     // at most VEC - 1 wraps
     auto slot_pos = [&](uint32_t col, int64_t row, int j, uint32_t* c, int64_t* r) {
@@ -128,26 +122,59 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
     }
     // keep the loads above ahead of the arithmetic below
     __builtin_amdgcn_sched_barrier(0);
+
+    // row (relative to brow) and column of vector k's first physical element
+    auto vec_pos = [&](int k, uint32_t* col, uint32_t* dr) {
+      const uint32_t cc = bcol + (uint32_t)(vidx(k) * VEC);
+      *dr = cc / inner32;
+      *col = cc - *dr * inner32;
+    };
+    // the ragged tile (the physically last) has padding: identity values,
+    // each a head of its own dead segment
+    const bool ragged = base + TILE > a.n;
+    // vector columns: one division per lane, then steps of 64 vectors
+    // (forward) or back (right to left) by the step's remainder
+    uint32_t col;
+    {
+      uint32_t dr0;
+      vec_pos(0, &col, &dr0);
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t e0 = base + vidx(k) * VEC;
-      uint32_t vcol;
-      int64_t vrow;
-      vec_pos(k, &vcol, &vrow);
-      T v[VEC];
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) v[j] = e0 + j < a.n ? raw[k][j] : id;
       uint32_t h = 0;
+      if (inner32 >= (uint32_t)VEC) {
+        // at most one row start in the vector: physical slot ps
+        const uint32_t ps = REVERSE ? inner32 - 1u - col : (col == 0 ? 0u : inner32 - col);
+        if (ps < (uint32_t)VEC) h = 1u << (REVERSE ? VEC - 1 - (int)ps : (int)ps);
+      } else {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const int pj = REVERSE ? VEC - 1 - j : j;  // physical slot of traversal slot j
-        uint32_t c;
-        int64_t r;
-        slot_pos(vcol, vrow, pj, &c, &r);
-        const bool pad = e0 + pj >= a.n;
-        h |= (pad || c == c_start ? 1u : 0u) << j;
-        x[k][j] = v[pj];
+        for (int j = 0; j < VEC; ++j) {
+          const int pj = REVERSE ? VEC - 1 - j : j;  // physical slot of traversal slot j
+          uint32_t c;
+          int64_t r;
+          slot_pos(col, 0, pj, &c, &r);
+          h |= (c == c_start ? 1u : 0u) << j;
+        }
       }
+      if (!REVERSE) {
+        col += a.step_r;
+        if (col >= inner32) col -= inner32;
+      } else {
+        const int c = (int)col - (int)a.step_r;
+        col = (uint32_t)(c < 0 ? c + (int)inner32 : c);
+      }
+      if (ragged) {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {
+          const int pj = REVERSE ? VEC - 1 - j : j;
+          const bool pad = e0 + pj >= a.n;
+          h |= (pad ? 1u : 0u) << j;
+          if (pad) raw[k][pj] = id;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) x[k][j] = raw[k][REVERSE ? VEC - 1 - j : j];
       hb[k] = h;
 #pragma unroll
       for (int j = 1; j < VEC; ++j) x[k][j] = (h >> j & 1u) ? x[k][j] : op(x[k][j - 1], x[k][j]);
@@ -201,9 +228,6 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
     for (int k = 0; k < K; ++k) {
       const S P = sop(XW, pre[k]);  // everything before this vector, segmented
       const int64_t e0 = base + vidx(k) * VEC;
-      uint32_t vcol;
-      int64_t vrow;
-      vec_pos(k, &vcol, &vrow);
       T y[VEC];  // by physical slot
       bool headed = false;  // a row start at or before traversal slot j in this vector
 #pragma unroll
@@ -222,15 +246,18 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
         }
         const int pj = REVERSE ? VEC - 1 - j : j;
         y[pj] = val;
-        // row extras: the total at the row's last traversal element, the seed
-        // slot at its first
-        const int64_t p = e0 + pj;
-        if (p < a.n) {
-          uint32_t c;
-          int64_t r;
-          slot_pos(vcol, vrow, pj, &c, &r);
-          if (a.total_out && c == c_end) a.total_out[r * a.total_stride] = a.has_seed ? op(a.seed, incl) : incl;
-          if (a.seed_out && c == c_start) a.seed_out[r * a.seed_stride] = a.seed;
+        // row extras (scan' totals, scanr z's seed slot): the total at the
+        // row's last traversal element, the seed at its first
+        if (a.total_out || a.seed_out) {
+          const int64_t p = e0 + pj;
+          if (p < a.n) {
+            uint32_t vc, vd, c;
+            int64_t r;
+            vec_pos(k, &vc, &vd);
+            slot_pos(vc, brow + vd, pj, &c, &r);
+            if (a.total_out && c == c_end) a.total_out[r * a.total_stride] = a.has_seed ? op(a.seed, incl) : incl;
+            if (a.seed_out && c == c_start) a.seed_out[r * a.seed_stride] = a.seed;
+          }
         }
       }
       if (ALIGNED && a.out_row == a.inner && e0 + VEC <= a.n) {
@@ -243,9 +270,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_rows(RowScanArgs<T> a) {
         for (int j = 0; j < VEC; ++j) {
           const int64_t p = e0 + j;
           if (p < a.n) {
-            uint32_t c;
+            uint32_t vc, vd, c;
             int64_t r;
-            slot_pos(vcol, vrow, j, &c, &r);
+            vec_pos(k, &vc, &vd);
+            slot_pos(vc, brow + vd, j, &c, &r);
             a.out[r * a.out_row + c] = y[j];
           }
         }
@@ -275,6 +303,7 @@ int scan_rows_launch(acc_stream_t s, RowScanArgs<T> a) {
   uint64_t* status;
   ACC_CHECK(stream_workspace(s, (size_t)ntiles * NW * 8, &ctl, &status));
   a.ntiles = (uint32_t)ntiles;
+  a.step_r = (uint32_t)(64 * Vec<T>::N % a.inner);
   a.ctl = ctl;
   a.status = status;
   auto kern = k_scan_rows<T, Op, BLOCK, K, REVERSE, ALIGNED>;

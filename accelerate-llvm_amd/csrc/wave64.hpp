@@ -23,6 +23,7 @@ enum : int {
   DPP_ROW_SHR8 = 0x118,
   DPP_ROW_BCAST15 = 0x142,
   DPP_ROW_BCAST31 = 0x143,
+  DPP_WAVE_SHR1 = 0x138,
 };
 
 template <typename T> struct Words {
@@ -80,6 +81,52 @@ __device__ __forceinline__ T wave_inclusive_scan(T x, Op op) {
   { T y = dpp_move<DPP_ROW_SHR8>(x); if (r >= 8) x = op(y, x); }
   { T y = dpp_move<DPP_ROW_BCAST15, 0xA>(x); if (lane & 16) x = op(y, x); }
   { T y = dpp_move<DPP_ROW_BCAST31, 0xC>(x); if (lane & 32) x = op(y, x); }
+  return x;
+}
+
+// A DPP lane move whose result is used only in lanes with an in-range
+// source (a row shift within its row, a row broadcast into its row mask):
+// the other lanes' values are unspecified, so no copy of the source is kept.
+template <int CTRL, int ROW_MASK = 0xF, int BANK_MASK = 0xF, typename T>
+__device__ __forceinline__ T dpp_fetch(const T& v) {
+  Words<T> a = to_words(v), r;
+#pragma unroll
+  for (int i = 0; i < Words<T>::N; ++i)
+    r.w[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)a.w[i], CTRL, ROW_MASK, BANK_MASK, true);
+  return from_words<T>(r);
+}
+
+// Value of lane (l - 1) by a wave-wide DPP shift; lane 0 receives fill.
+template <typename T> __device__ __forceinline__ T wave_shr1(const T& v, const T& fill) {
+  Words<T> a = to_words(v), f = to_words(fill), r;
+#pragma unroll
+  for (int i = 0; i < Words<T>::N; ++i)
+    r.w[i] = (uint32_t)__builtin_amdgcn_update_dpp((int)f.w[i], (int)a.w[i], DPP_WAVE_SHR1, 0xF, 0xF, false);
+  return from_words<T>(r);
+}
+
+// Segmented inclusive scan over the 64 lanes, lane 0 first.  A segment
+// starts at each lane whose bit is set in heads (a wave-uniform ballot);
+// lane i combines lanes [max(L, 0), i] where L is the last head at or below
+// i.  Each lane first finds its distance s to L (lane + 1 when there is
+// none); the lane-shift step over distance d then applies exactly when
+// d <= s, so the steps move only values, never flags.  below: a head lies
+// strictly below this lane.
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_seg_inclusive_scan(T x, uint64_t heads, Op op, bool* below) {
+  const int lane = lane_id();
+  const int r = lane & 15;
+  const uint64_t lt = heads & ((1ull << lane) - 1ull);  // heads strictly below
+  const bool me = (heads >> lane) & 1ull;
+  *below = lt != 0;
+  const int s = me ? 0 : (lt ? lane - (63 - __builtin_clzll(lt)) : lane + 1);
+  const int rs = min(r, s);
+  { T y = dpp_fetch<DPP_ROW_SHR1>(x); if (rs >= 1) x = op(y, x); }
+  { T y = dpp_fetch<DPP_ROW_SHR2>(x); if (rs >= 2) x = op(y, x); }
+  { T y = dpp_fetch<DPP_ROW_SHR4>(x); if (rs >= 4) x = op(y, x); }
+  { T y = dpp_fetch<DPP_ROW_SHR8>(x); if (rs >= 8) x = op(y, x); }
+  { T y = dpp_fetch<DPP_ROW_BCAST15, 0xA>(x); if ((lane & 16) && s > r) x = op(y, x); }
+  { T y = dpp_fetch<DPP_ROW_BCAST31, 0xC>(x); if ((lane & 32) && s > lane - 32) x = op(y, x); }
   return x;
 }
 

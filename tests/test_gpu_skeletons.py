@@ -211,7 +211,8 @@ def test_scan_rows(stream, shape):
         assert np.array_equal(e, we) and np.array_equal(s, ws)
 
 
-ROW_SHAPES = [(1000, 3), (4096, 17), (300, 1), (9, 1000), (3, 100003), (2, 1 << 20), (70000, 2)]
+ROW_SHAPES = [(1000, 3), (4096, 17), (300, 1), (9, 1000), (3, 100003), (2, 1 << 20), (70000, 2),
+              (2048, 4), (999, 5), (777, 63), (513, 64), (300, 65), (33, 8191)]
 
 
 @pytest.mark.parametrize("shape", ROW_SHAPES)

@@ -178,7 +178,8 @@ def test_reverse(stream, dt, n):
     same(o.to_numpy(stream), O.transform(O.IX_REVERSE, O.UNOP_ABS, x, 1, n))
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 100), (100, 1), (63, 65), (64, 64), (129, 257), (1000, 999)])
+@pytest.mark.parametrize("shape", [(1, 1), (1, 100), (100, 1), (63, 65), (64, 64), (129, 257), (1000, 999),
+                                   (128, 200), (100, 68), (192, 64), (256, 1024), (512, 256)])
 @pytest.mark.parametrize("dt", [np.int32, np.float64, np.uint64, np.float32])
 def test_transpose(stream, dt, shape):
     r, c = shape
@@ -186,6 +187,26 @@ def test_transpose(stream, dt, shape):
     o = A.DeviceArray(r * c, dt)
     A.transform(stream, A.IX_TRANSPOSE, A.UNOP_ID, dev(stream, x), r, c, None, r * c, o)
     same(o.to_numpy(stream), O.transform(O.IX_TRANSPOSE, O.UNOP_ID, x, r, c))
+
+
+def test_transpose_views_and_identity(stream):
+    # a view one element in (rows no longer 16-byte aligned: element path)
+    r, c = 128, 256
+    x = vals(np.int32, r * c + 1, 3)
+    d = dev(stream, x)
+    o = A.DeviceArray(r * c, np.int32)
+    A.transform(stream, A.IX_TRANSPOSE, A.UNOP_NEG, d.view(1, r * c), r, c, None, r * c, o)
+    same(o.to_numpy(stream), O.transform(O.IX_TRANSPOSE, O.UNOP_NEG, x[1:], r, c))
+    # transposing twice is the identity (vector path, 4096 x 2048 Float64)
+    r, c = 4096, 2048
+    y = vals(np.float64, r * c, 4)
+    dy = dev(stream, y)
+    t = A.DeviceArray(r * c, np.float64)
+    A.transform(stream, A.IX_TRANSPOSE, A.UNOP_ID, dy, r, c, None, r * c, t)
+    back = A.DeviceArray(r * c, np.float64)
+    A.transform(stream, A.IX_TRANSPOSE, A.UNOP_ID, t, c, r, None, r * c, back)
+    same(back.to_numpy(stream), y)
+    same(t.to_numpy(stream), y.reshape(r, c).T.reshape(-1))
 
 
 def test_producer_errors(stream):

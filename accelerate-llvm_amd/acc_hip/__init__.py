@@ -185,6 +185,8 @@ SIGNATURES = {
                                   c_int64, c_void_p]),
     "acc_hip_stencil3d": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p,
                                   c_int64, c_int64, c_int64, c_void_p]),
+    "acc_hip_stencil2_2d": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int64,
+                                    c_int64, c_void_p, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     "acc_hip_map": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "acc_hip_zipwith": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "acc_hip_generate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
@@ -408,6 +410,20 @@ def stencil2d(stream, ry, rx, weights, boundary, constant, src, height, width, o
     keep, cp = _scalar(dt, constant)
     check(lib().acc_hip_stencil2d(stream.handle, acc_type(dt), ry, rx, w.ctypes.data_as(c_void_p), boundary, cp,
                                   _p(src), height, width, _p(out)))
+
+
+def stencil2_2d(stream, ry, rx, weights1, boundary1, constant1, src1, height1, width1,
+                weights2, boundary2, constant2, src2, height2, width2, out):
+    """stencil2 on DIM2 arrays: the weighted sums of both inputs' neighbourhoods
+    over the intersection of their shapes, each input with its own boundary."""
+    dt = src1.dtype
+    w1 = np.ascontiguousarray(np.asarray(weights1, dtype=dt).reshape(-1))
+    w2 = np.ascontiguousarray(np.asarray(weights2, dtype=dt).reshape(-1))
+    k1, c1 = _scalar(dt, constant1)
+    k2, c2 = _scalar(dt, constant2)
+    check(lib().acc_hip_stencil2_2d(stream.handle, acc_type(dt), ry, rx, w1.ctypes.data_as(c_void_p), boundary1, c1,
+                                    _p(src1), height1, width1, w2.ctypes.data_as(c_void_p), boundary2, c2, _p(src2),
+                                    height2, width2, _p(out)))
 
 
 def stencil3d(stream, rz, ry, rx, weights, boundary, constant, src, depth, height, width, out):

@@ -507,6 +507,12 @@ def main():
                timed(s, lambda: A.stencil2d(s, 2, 2, w5, A.CLAMP, None, d_img, H, WD, d_out), R))
         record("stencil3x3_mirror_f32_8192^2", 2 * H * WD * 4,
                timed(s, lambda: A.stencil2d(s, 1, 1, w3, A.MIRROR, None, d_img, H, WD, d_out), R))
+        # stencil2: the image (3x3, clamp) minus its blur (3x3, wrap)
+        d_out2 = A.DeviceArray(H * WD, np.float32)
+        record("stencil2_3x3_clamp+wrap_f32_8192^2", 3 * H * WD * 4,
+               timed(s, lambda: A.stencil2_2d(s, 1, 1, w3, A.CLAMP, None, d_img, H, WD, -w3, A.WRAP, None, d_out,
+                                              H, WD, d_out2), R))
+        del d_out2
         # 3-D: 3x3x3 box, clamp, on 512^3 Float32 (2^27 elements)
         n3 = 512
         d_vol = A.DeviceArray(n3 ** 3, np.float32)

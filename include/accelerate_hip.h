@@ -243,6 +243,22 @@ int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, const void* we
                       acc_boundary boundary, const void* constant, const void* in,
                       int64_t height, int64_t width, void* out);
 
+/* stencil2 on DIM2 arrays (LLVM/Execute.hs:147-156, called at :326; stencil2Op,
+ * PTX/Execute.hs:669-684; NAT/CodeGen/Stencil.hs:75-107) with the stencil function
+ * f(s1, s2) = sum over taps of w1[dy][dx] * in1[y+dy][x+dx] + sum over taps of
+ * w2[dy][dx] * in2[y+dy][x+dx], taps in row-major order, in1's before in2's.  The
+ * output is height x width with height = min(height1, height2) and width =
+ * min(width1, width2) (the intersection, PTX/Execute.hs:682); each input is read
+ * through its own boundary against its own extents.  Both stencils have radii
+ * ry, rx in 1..2 (write a smaller one with zero weights); weights1/weights2 are
+ * host arrays of (2ry+1)*(2rx+1) elements of type t.  The output must not alias
+ * an input. */
+int acc_hip_stencil2_2d(acc_stream_t s, acc_type t, int ry, int rx, const void* weights1,
+                        acc_boundary boundary1, const void* constant1, const void* in1,
+                        int64_t height1, int64_t width1, const void* weights2,
+                        acc_boundary boundary2, const void* constant2, const void* in2,
+                        int64_t height2, int64_t width2, void* out);
+
 /* stencil1 on a DIM3 array (LLVM/Execute.hs:131-138; PTX/Execute.hs:652-776) with
  * f = sum over taps of w[dz][dy][dx] * x[z+dz][y+dy][x+dx], taps in tuple order
  * (planes, rows, columns), dz in [-rz, rz], dy in [-ry, ry], dx in [-rx, rx];

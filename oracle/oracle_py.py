@@ -317,6 +317,32 @@ def stencil2d_f64(ry, rx, weights, boundary, constant, img):
     return out, ab
 
 
+def _crop2(a, b):
+    h, w = min(a.shape[0], b.shape[0]), min(a.shape[1], b.shape[1])
+    return a[:h, :w], b[:h, :w]
+
+
+def stencil2_2d(ry, rx, w1, b1, c1, img1, w2, b2, c2, img2):
+    """stencil2 on DIM2 arrays with the weighted-sum function (stencil2Op,
+    PTX/Execute.hs:669-684; NAT/CodeGen/Stencil.hs:75-107): each input's
+    stencil over its own shape through its own boundary, both cut to the
+    intersection of the shapes (PTX/Execute.hs:682) and added in the element
+    type (integers wrap; floating sums round differently from a single
+    accumulator: use stencil2_2d_f64 for them)."""
+    s1, s2 = _crop2(stencil2d(ry, rx, w1, b1, c1, img1), stencil2d(ry, rx, w2, b2, c2, img2))
+    return s1 + s2
+
+
+def stencil2_2d_f64(ry, rx, w1, b1, c1, img1, w2, b2, c2, img2):
+    """FP64 truth of stencil2_2d for Float32 inputs and the sum of |w * x| per
+    output (for the rounding bound)."""
+    t1, a1 = stencil2d_f64(ry, rx, w1, b1, c1, img1)
+    t2, a2 = stencil2d_f64(ry, rx, w2, b2, c2, img2)
+    t1, t2 = _crop2(t1, t2)
+    a1, a2 = _crop2(a1, a2)
+    return t1 + t2, a1 + a2
+
+
 def divide_work1(splits, minsize, lo, hi):
     cap = 4 * max(splits, 1) + 16
     r = np.empty(2 * cap, np.int64)

@@ -570,6 +570,79 @@ def test_stencil_int_large_image(stream, r):
         assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(ry, rx, wt, b, np.int32(7), img)), b
 
 
+# ------------------------------------------------------------- stencil2
+S2_SHAPES = [((130, 211), (130, 211)), ((300, 512), (300, 512)), ((200, 516), (190, 640)),
+             ((37, 70), (41, 66)), ((5, 3), (3, 7)), ((1, 1), (2, 2))]
+
+
+@pytest.mark.parametrize("r", [(1, 1), (1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("shapes", S2_SHAPES)
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+def test_stencil2_int_exact(stream, dt, shapes, r):
+    # integer stencil2 is exact (wrapping): equal and unequal shapes, 16-byte
+    # rows (vector-staged interior tiles) and others, every boundary pair
+    ry, rx = r
+    (h1, w1), (h2, w2) = shapes
+    rng = np.random.default_rng(h1 * 7 + w2)
+    a = rng.integers(-1000, 1000, size=(h1, w1)).astype(dt)
+    b = rng.integers(-1000, 1000, size=(h2, w2)).astype(dt)
+    wa = rng.integers(-3, 4, size=(2 * ry + 1, 2 * rx + 1)).astype(dt)
+    wb = rng.integers(-3, 4, size=(2 * ry + 1, 2 * rx + 1)).astype(dt)
+    h, w = min(h1, h2), min(w1, w2)
+    da, db = dev(stream, a), dev(stream, b)
+    out = A.DeviceArray(h * w, dt)
+    for b1, b2 in ((CLAMP, WRAP), (MIRROR, CONSTANT), (CONSTANT, CLAMP), (WRAP, MIRROR)):
+        if (b1 == MIRROR and (h1 <= ry or w1 <= rx)) or (b2 == MIRROR and (h2 <= ry or w2 <= rx)):
+            continue  # mirror needs extent > radius (the reference maps once)
+        A.stencil2_2d(stream, ry, rx, wa, b1, dt(5), da, h1, w1, wb, b2, dt(-9), db, h2, w2, out)
+        want = O.stencil2_2d(ry, rx, wa, b1, dt(5), a, wb, b2, dt(-9), b)
+        assert np.array_equal(out.to_numpy(stream).reshape(h, w), want), (b1, b2)
+
+
+@pytest.mark.parametrize("r", [(1, 1), (2, 2)])
+@pytest.mark.parametrize("shapes", [((256, 1024), (256, 1024)), ((131, 515), (140, 500))])
+def test_stencil2_float32(stream, shapes, r):
+    ry, rx = r
+    (h1, w1), (h2, w2) = shapes
+    a = O.gen_normal_f32(h1 + w1, h1 * w1).reshape(h1, w1)
+    b = O.gen_normal_f32(h2 + w2 + 1, h2 * w2).reshape(h2, w2)
+    rng = np.random.default_rng(ry * 10 + rx)
+    wa = rng.standard_normal((2 * ry + 1, 2 * rx + 1)).astype(np.float32)
+    wb = rng.standard_normal((2 * ry + 1, 2 * rx + 1)).astype(np.float32)
+    h, w = min(h1, h2), min(w1, w2)
+    out = A.DeviceArray(h * w, np.float32)
+    A.stencil2_2d(stream, ry, rx, wa, CLAMP, None, dev(stream, a), h1, w1, wb, CONSTANT, 0.5, dev(stream, b), h2, w2,
+                  out)
+    got = out.to_numpy(stream).reshape(h, w).astype(np.float64)
+    truth, ab = O.stencil2_2d_f64(ry, rx, wa, CLAMP, None, a, wb, CONSTANT, 0.5, b)
+    taps = 2 * (2 * ry + 1) * (2 * rx + 1)
+    assert np.all(np.abs(got - truth) <= taps * 2.0 ** -23 * ab + 1e-30)
+
+
+def test_stencil2_large_and_errors(stream):
+    h, w = 2048, 4096
+    rng = np.random.default_rng(5)
+    a = rng.integers(-1000, 1000, size=(h, w)).astype(np.int32)
+    b = rng.integers(-1000, 1000, size=(h, w)).astype(np.int32)
+    wa = np.outer([1, 2, 1], [1, 2, 1]).astype(np.int32)
+    wb = np.outer([1, 4, 6, 4, 1], [1, 0, -1]).astype(np.int32).T.copy()  # 3 rows x 5 columns
+    wb = wb.reshape(3, 5)
+    wa5 = np.zeros((3, 5), np.int32)
+    wa5[:, 1:4] = wa
+    da, db = dev(stream, a), dev(stream, b)
+    out = A.DeviceArray(h * w, np.int32)
+    A.stencil2_2d(stream, 1, 2, wa5, MIRROR, None, da, h, w, wb, WRAP, None, db, h, w, out)
+    assert np.array_equal(out.to_numpy(stream).reshape(h, w),
+                          O.stencil2_2d(1, 2, wa5, MIRROR, None, a, wb, WRAP, None, b))
+    with pytest.raises(A.AccError):  # radius 3
+        A.stencil2_2d(stream, 3, 1, np.zeros(21, np.int32), CLAMP, None, da, h, w, np.zeros(21, np.int32), CLAMP,
+                      None, db, h, w, out)
+    with pytest.raises(A.AccError):  # the output aliases an input
+        A.stencil2_2d(stream, 1, 1, wa, CLAMP, None, da, h, w, wa, CLAMP, None, db, h, w, da)
+    with pytest.raises(A.AccError):  # constant boundary without a value
+        A.stencil2_2d(stream, 1, 1, wa, CONSTANT, None, da, h, w, wa, CLAMP, None, db, h, w, out)
+
+
 def test_stencil_float64_special(stream):
     h, w = 70, 90
     img = np.full((h, w), 3.0)

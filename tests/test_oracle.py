@@ -259,6 +259,45 @@ def test_integer_stencil_exact():
     assert np.array_equal(out.astype(np.int64), want)
 
 
+def test_stencil2_against_scipy_and_linearity():
+    # stencil2 (two inputs, own boundaries, intersection of the shapes) against
+    # scipy's correlate of each input, and its reductions: zero weights on one
+    # side give the other side's stencil1; one array twice (same boundary)
+    # gives stencil1 with the summed weights
+    rng = np.random.default_rng(21)
+    a = rng.integers(-1000, 1000, size=(23, 31)).astype(np.int32)
+    b = rng.integers(-1000, 1000, size=(27, 29)).astype(np.int32)
+    w1 = rng.integers(-3, 4, size=(3, 5)).astype(np.int32)
+    w2 = rng.integers(-3, 4, size=(3, 5)).astype(np.int32)
+    got = O.stencil2_2d(1, 2, w1, CLAMP, None, a, w2, WRAP, None, b)
+    assert got.shape == (23, 29)
+    want = (_scipy_correlate(a, w1, CLAMP)[:23, :29].astype(np.int64) +
+            _scipy_correlate(b, w2, WRAP)[:23, :29].astype(np.int64))
+    assert np.array_equal(got.astype(np.int64), want)
+    z = np.zeros_like(w2)
+    assert np.array_equal(O.stencil2_2d(1, 2, w1, MIRROR, None, a, z, CLAMP, None, b),
+                          O.stencil2d(1, 2, w1, MIRROR, None, a)[:23, :29])
+    assert np.array_equal(O.stencil2_2d(1, 2, w1, MIRROR, None, a, w2, MIRROR, None, a),
+                          O.stencil2d(1, 2, w1 + w2, MIRROR, None, a))
+
+
+def test_stencil2_each_input_its_own_extents():
+    # a is 2 x 3, b is 3 x 2: the output is 2 x 2.  Below the output's last
+    # row, b still has a row of its own (no boundary applies there); a is
+    # clamped at its own last row
+    a = np.arange(6, dtype=np.float64).reshape(2, 3)       # [[0,1,2],[3,4,5]]
+    b = 10 * np.arange(6, dtype=np.float64).reshape(3, 2)  # [[0,10],[20,30],[40,50]]
+    down = np.zeros((3, 3))
+    down[2, 1] = 1.0  # the tap at y + 1
+    z = np.zeros((3, 3))
+    assert np.array_equal(O.stencil2_2d(1, 1, z, CLAMP, None, a, down, CLAMP, None, b),
+                          np.array([[20.0, 30.0], [40.0, 50.0]]))
+    assert np.array_equal(O.stencil2_2d(1, 1, down, CLAMP, None, a, z, CLAMP, None, b),
+                          np.array([[3.0, 4.0], [3.0, 4.0]]))
+    assert np.array_equal(O.stencil2_2d(1, 1, down, CONSTANT, -1.0, a, down, CONSTANT, -2.0, b),
+                          np.array([[23.0, 34.0], [39.0, 49.0]]))
+
+
 # --------------------------------------------------------------- schedule
 def test_divide_work1_pieces():
     for splits, minsize, n in ((7, 4096, 1 << 20), (3, 8192, 100000), (15, 4096, 5000), (0, 4096, 123)):

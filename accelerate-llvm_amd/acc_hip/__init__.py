@@ -26,7 +26,7 @@ ADD, MUL, MIN, MAX, AND, OR, XOR, SUB, EXCHANGE = range(9)
 # acc_direction
 LEFT_TO_RIGHT, RIGHT_TO_LEFT = 0, 1
 # acc_boundary
-CLAMP, MIRROR, WRAP, CONSTANT = range(4)
+CLAMP, MIRROR, WRAP, CONSTANT, FUNCTION = range(5)
 # acc_unop (map / transform element functions)
 UNOP_ID, UNOP_NEG, UNOP_ABS, UNOP_AFFINE, UNOP_CAST = range(5)
 # acc_genop
@@ -364,9 +364,10 @@ class DeviceView:
 
 
 def _scalar(dtype, v):
+    """A scalar, or a sequence (the function boundary's c0, cy, cx), as a C array."""
     if v is None:
         return None, None
-    a = np.array([v], dtype=dtype)
+    a = np.ascontiguousarray(np.array(v, dtype=dtype).reshape(-1))
     return a, a.ctypes.data_as(c_void_p)
 
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "../../include/accelerate_hip.h"
 
 namespace acc {
@@ -22,6 +23,22 @@ __device__ __forceinline__ int64_t bound_index(int b, int64_t i, int64_t n) {
   }
   // arrays smaller than the radius: keep the single-step map in bounds
   return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+// The Function boundary's value at a 2-D index (y, x) outside the array
+// (LLVM/CodeGen/Stencil.hs:215-218) for the affine function set:
+// (c0 + cy*y) + cx*x in the element type (integers wrap, floating operations
+// round one by one).  The callers keep extents below 2^31 - 8 for it, so the
+// halo's coordinates are 32-bit (one conversion instruction each).
+template <typename T> __device__ __forceinline__ T bound_function(const T (&c)[3], int32_t y, int32_t x) {
+#pragma clang fp contract(off)
+  if constexpr (std::is_integral<T>::value) {
+    using U = typename std::make_unsigned<T>::type;
+    return (T)((U)((U)c[0] + (U)c[1] * (U)(T)y) + (U)c[2] * (U)(T)x);
+  } else {
+    const T t = c[0] + c[1] * (T)y;
+    return t + c[2] * (T)x;
+  }
 }
 
 }  // namespace acc

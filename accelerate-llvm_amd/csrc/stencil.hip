@@ -40,6 +40,7 @@ template <typename T> struct StArgs {
   uint32_t ix0, ix1, iy0, iy1;
   int boundary;
   T cval;
+  T fc[3];   // Function boundary: (c0 + cy*y) + cx*x
   T wt[81];  // (2RY+1) x (2RX+1) weights, row-major
 };
 
@@ -152,6 +153,19 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
     }
   }
   __syncthreads();
+  if (!interior && a.boundary == ACC_BOUNDARY_FUNCTION) {
+    // the function boundary: a second pass gives each cell outside the array
+    // f(index) (staged above from a clamped index), so the staging keeps the
+    // registers of the other boundaries
+    constexpr int TOT = LH * LW;
+    for (int i = threadIdx.x; i < TOT; i += ST_BLOCK) {
+      const int r = i / LW, c = i - r * LW;
+      const int64_t gy = y0 - RY + r, gx = x0 - RX + c;
+      if ((uint64_t)gy >= (uint64_t)a.h || (uint64_t)gx >= (uint64_t)a.w)
+        tile[r * LS + c + CO] = bound_function(a.fc, (int32_t)gy, (int32_t)gx);
+    }
+    __syncthreads();
+  }
 
   // ---- 4x4 outputs per thread from a register window
   const int px = threadIdx.x % (ST_TW / ST_PX), py = threadIdx.x / (ST_TW / ST_PX);
@@ -234,7 +248,8 @@ int stencil_launch(acc_stream_t s, const T* w, int boundary, const T* cval, cons
   a.iy0 = (uint32_t)iy0;
   a.iy1 = (uint32_t)iy1;
   a.boundary = boundary;
-  a.cval = cval ? *cval : T();
+  a.cval = cval && boundary == ACC_BOUNDARY_CONSTANT ? *cval : T();
+  for (int k = 0; k < 3; ++k) a.fc[k] = cval && boundary == ACC_BOUNDARY_FUNCTION ? cval[k] : T();
   const int taps = (2 * RY + 1) * (2 * RX + 1);
   for (int i = 0; i < 81; ++i) a.wt[i] = i < taps ? w[i] : T();
   const int64_t nt = tiles_x * tiles_y;
@@ -278,10 +293,12 @@ extern "C" int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, con
   if (!s) return fail(ACC_ERROR_INVALID_VALUE, "stencil: null stream");
   if (height < 0 || width < 0) return fail(ACC_ERROR_INVALID_VALUE, "stencil: negative extent");
   if (ry < 1 || ry > 4 || rx < 1 || rx > 4) return fail(ACC_ERROR_UNSUPPORTED, "stencil: radius must be 1..4");
-  if (boundary < ACC_BOUNDARY_CLAMP || boundary > ACC_BOUNDARY_CONSTANT)
+  if (boundary < ACC_BOUNDARY_CLAMP || boundary > ACC_BOUNDARY_FUNCTION)
     return fail(ACC_ERROR_INVALID_VALUE, "stencil: boundary %d", (int)boundary);
-  if (boundary == ACC_BOUNDARY_CONSTANT && !constant)
-    return fail(ACC_ERROR_INVALID_VALUE, "stencil: constant boundary needs a value");
+  if ((boundary == ACC_BOUNDARY_CONSTANT || boundary == ACC_BOUNDARY_FUNCTION) && !constant)
+    return fail(ACC_ERROR_INVALID_VALUE, "stencil: constant/function boundary needs its values");
+  if (boundary == ACC_BOUNDARY_FUNCTION && (height >= 0x7ffffff8LL || width >= 0x7ffffff8LL))
+    return fail(ACC_ERROR_UNSUPPORTED, "stencil: function boundary on extents of 2^31 or more");
   if (!weights) return fail(ACC_ERROR_INVALID_VALUE, "stencil: null weights");
   if (height == 0 || width == 0) return ACC_SUCCESS;
   if (!in || !out) return fail(ACC_ERROR_INVALID_VALUE, "stencil: null array");

@@ -38,6 +38,7 @@ template <typename T> struct St2Src {
   int64_t h, w;  // extents; w is also the row pitch
   int boundary;
   T cval;
+  T fc[3];   // Function boundary: (c0 + cy*y) + cx*x
   T wt[25];  // (2RY+1) x (2RX+1) weights, row-major (radii 1..2)
 };
 
@@ -162,6 +163,23 @@ __global__ __launch_bounds__(S2_BLOCK) void k_stencil2(St2Args<T> a) {
     }
   }
   __syncthreads();
+  if (!interior && (a.src[0].boundary == ACC_BOUNDARY_FUNCTION || a.src[1].boundary == ACC_BOUNDARY_FUNCTION)) {
+    // the function boundary: a second pass gives each cell outside that
+    // input f(index) (staged above from a clamped index), so the staging
+    // keeps the registers of the other boundaries
+    constexpr int TOT = LH * LW;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (a.src[s].boundary != ACC_BOUNDARY_FUNCTION) continue;
+      for (int i = threadIdx.x; i < TOT; i += S2_BLOCK) {
+        const int r = i / LW, c = i - r * LW;
+        const int64_t gy = y0 - RY + r, gx = x0 - RX + c;
+        if ((uint64_t)gy >= (uint64_t)a.src[s].h || (uint64_t)gx >= (uint64_t)a.src[s].w)
+          tile[s][r * LS + c + CO] = bound_function(a.src[s].fc, (int32_t)gy, (int32_t)gx);
+      }
+    }
+    __syncthreads();
+  }
 
   // ---- 4x2 outputs per thread: a's taps, then b's, in tuple order
   const int px = threadIdx.x % (S2_TW / S2_PX), py = threadIdx.x / (S2_TW / S2_PX);
@@ -256,7 +274,9 @@ int stencil2_any(acc_stream_t s, int ry, int rx, const void* const* weights, con
     src.h = height[k];
     src.w = width[k];
     src.boundary = (int)boundary[k];
-    src.cval = constant[k] ? *(const T*)constant[k] : T();
+    const T* cv = (const T*)constant[k];
+    src.cval = cv && boundary[k] == ACC_BOUNDARY_CONSTANT ? cv[0] : T();
+    for (int j = 0; j < 3; ++j) src.fc[j] = cv && boundary[k] == ACC_BOUNDARY_FUNCTION ? cv[j] : T();
     for (int i = 0; i < 25; ++i) src.wt[i] = i < taps ? ((const T*)weights[k])[i] : T();
   }
   a.out = (T*)out;
@@ -282,10 +302,13 @@ extern "C" int acc_hip_stencil2_2d(acc_stream_t s, acc_type t, int ry, int rx, c
   const acc_boundary bnd[2] = {boundary1, boundary2};
   const void* cst[2] = {constant1, constant2};
   for (int k = 0; k < 2; ++k) {
-    if (bnd[k] < ACC_BOUNDARY_CLAMP || bnd[k] > ACC_BOUNDARY_CONSTANT)
+    if (bnd[k] < ACC_BOUNDARY_CLAMP || bnd[k] > ACC_BOUNDARY_FUNCTION)
       return fail(ACC_ERROR_INVALID_VALUE, "stencil2: boundary %d", (int)bnd[k]);
-    if (bnd[k] == ACC_BOUNDARY_CONSTANT && !cst[k])
-      return fail(ACC_ERROR_INVALID_VALUE, "stencil2: constant boundary needs a value");
+    if ((bnd[k] == ACC_BOUNDARY_CONSTANT || bnd[k] == ACC_BOUNDARY_FUNCTION) && !cst[k])
+      return fail(ACC_ERROR_INVALID_VALUE, "stencil2: constant/function boundary needs its values");
+    if (bnd[k] == ACC_BOUNDARY_FUNCTION && (std::min(height1, height2) >= 0x7ffffff8LL ||
+                                            std::min(width1, width2) >= 0x7ffffff8LL))
+      return fail(ACC_ERROR_UNSUPPORTED, "stencil2: function boundary on extents of 2^31 or more");
   }
   if (!weights1 || !weights2) return fail(ACC_ERROR_INVALID_VALUE, "stencil2: null weights");
   if (std::min(height1, height2) == 0 || std::min(width1, width2) == 0) return ACC_SUCCESS;

@@ -222,6 +222,8 @@ extern "C" int acc_hip_stencil3d(acc_stream_t s, acc_type t, int rz, int ry, int
   if (depth < 0 || height < 0 || width < 0) return fail(ACC_ERROR_INVALID_VALUE, "stencil3d: negative extent");
   if (rz < 1 || rz > 2 || ry < 1 || ry > 2 || rx < 1 || rx > 2)
     return fail(ACC_ERROR_UNSUPPORTED, "stencil3d: radius must be 1 or 2");
+  if (boundary == ACC_BOUNDARY_FUNCTION)
+    return fail(ACC_ERROR_UNSUPPORTED, "stencil3d: the function boundary is for the 2-D stencils");
   if (boundary < ACC_BOUNDARY_CLAMP || boundary > ACC_BOUNDARY_CONSTANT)
     return fail(ACC_ERROR_INVALID_VALUE, "stencil3d: boundary %d", (int)boundary);
   if (boundary == ACC_BOUNDARY_CONSTANT && !constant)

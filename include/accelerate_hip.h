@@ -77,12 +77,20 @@ typedef enum acc_op {
 
 typedef enum acc_direction { ACC_LEFT_TO_RIGHT = 0, ACC_RIGHT_TO_LEFT = 1 } acc_direction;
 
-/* Stencil boundary conditions (LLVM/CodeGen/Stencil.hs:44-49, index maps :224-253). */
+/* Stencil boundary conditions (LLVM/CodeGen/Stencil.hs:44-49, index maps :224-253).
+ * ACC_BOUNDARY_FUNCTION is the Function boundary (an index outside the array
+ * reads f(index), :215-218) with the affine functions of the index as its
+ * function set: the `constant` argument then points to three values c0, cy, cx
+ * of the element type and a 2-D index (y, x) outside reads
+ * (c0 + cy*y) + cx*x, computed in the element type (y and x converted to it;
+ * integers wrap; floating operations round one by one, no fused multiply-add).
+ * The 2-D stencils take it; the 3-D stencil does not (ACC_ERROR_UNSUPPORTED). */
 typedef enum acc_boundary {
   ACC_BOUNDARY_CLAMP = 0,
   ACC_BOUNDARY_MIRROR = 1,
   ACC_BOUNDARY_WRAP = 2,
-  ACC_BOUNDARY_CONSTANT = 3
+  ACC_BOUNDARY_CONSTANT = 3,
+  ACC_BOUNDARY_FUNCTION = 4
 } acc_boundary;
 
 typedef struct acc_stream_s* acc_stream_t;
@@ -238,7 +246,7 @@ int acc_hip_permute(acc_stream_t s, acc_type t, acc_op combine, const void* defa
  * taps in row-major order (LLVM/CodeGen/Stencil.hs:126-180), dy in [-ry, ry],
  * dx in [-rx, rx]; `weights` is a host array of (2ry+1)*(2rx+1) elements of
  * type t.  Radii 1..4 (3/5/7/9-point patterns).  `constant` is used with
- * ACC_BOUNDARY_CONSTANT. */
+ * ACC_BOUNDARY_CONSTANT (one value) and ACC_BOUNDARY_FUNCTION (c0, cy, cx). */
 int acc_hip_stencil2d(acc_stream_t s, acc_type t, int ry, int rx, const void* weights,
                       acc_boundary boundary, const void* constant, const void* in,
                       int64_t height, int64_t width, void* out);

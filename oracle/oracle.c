@@ -17,7 +17,7 @@
 
 enum { OP_ADD = 0, OP_MUL = 1, OP_MIN = 2, OP_MAX = 3, OP_AND = 4, OP_OR = 5, OP_XOR = 6,
        OP_SUB = 7, OP_EXCHANGE = 8 };
-enum { B_CLAMP = 0, B_MIRROR = 1, B_WRAP = 2, B_CONSTANT = 3 };
+enum { B_CLAMP = 0, B_MIRROR = 1, B_WRAP = 2, B_CONSTANT = 3, B_FUNCTION = 4 };
 
 /* ------------------------------------------------------------ combine */
 #define INT_COMBINE(T, U, N)                                                            \
@@ -174,7 +174,8 @@ static inline int64_t bmap(int b, int64_t i, int64_t n) {
   }                                                                                             \
   /* stencil1 with f = sum of w * x over the (2ry+1)x(2rx+1) taps in tuple order: rows      */  \
   /* (leftmost index) -ry..ry outer, columns -rx..rx inner (LLVM/CodeGen/Stencil.hs:54-180) */  \
-  /* and boundary maps at :207-253 (constant: any coordinate outside gives c).             */  \
+  /* and boundary maps at :207-253 (constant: any coordinate outside gives c; function:   */  \
+  /* an index outside gives f(index), :215-218, here f = (c0 + cy*y) + cx*x in T).        */  \
   int or_stencil2d_##N(int ry, int rx, const T* w, int boundary, const T* cval, const T* in,   \
                        int64_t h, int64_t wd, T* out) {                                         \
     for (int64_t y = 0; y < h; ++y)                                                             \
@@ -185,10 +186,18 @@ static inline int64_t bmap(int b, int64_t i, int64_t n) {
           for (int dx = -rx; dx <= rx; ++dx) {                                                  \
             const int64_t yy = y + dy, xx = x + dx;                                             \
             T v;                                                                                \
-            if (boundary == B_CONSTANT && (yy < 0 || yy >= h || xx < 0 || xx >= wd))            \
+            const int outside = yy < 0 || yy >= h || xx < 0 || xx >= wd;                       \
+            if (boundary == B_CONSTANT && outside) {                                            \
               v = *cval;                                                                        \
-            else                                                                                \
+            } else if (boundary == B_FUNCTION && outside) {                                     \
+              T fy, fx;                                                                         \
+              comb_##N(OP_MUL, cval[1], (T)yy, &fy);                                            \
+              comb_##N(OP_ADD, cval[0], fy, &fy);                                               \
+              comb_##N(OP_MUL, cval[2], (T)xx, &fx);                                            \
+              comb_##N(OP_ADD, fy, fx, &v);                                                     \
+            } else {                                                                            \
               v = in[bmap(boundary, yy, h) * wd + bmap(boundary, xx, wd)];                      \
+            }                                                                                   \
             T p;                                                                                \
             comb_##N(OP_MUL, w[(dy + ry) * (2 * rx + 1) + (dx + rx)], v, &p);                   \
             if (first) { acc = p; first = 0; } else comb_##N(OP_ADD, acc, p, &acc);             \
@@ -253,8 +262,15 @@ void or_stencil2d_f32_f64(int ry, int rx, const float* w, int boundary, const fl
         for (int dx = -rx; dx <= rx; ++dx) {
           const int64_t yy = y + dy, xx = x + dx;
           double v;
-          if (boundary == B_CONSTANT && (yy < 0 || yy >= h || xx < 0 || xx >= wd)) v = *cval;
-          else v = in[bmap(boundary, yy, h) * wd + bmap(boundary, xx, wd)];
+          const int outside = yy < 0 || yy >= h || xx < 0 || xx >= wd;
+          if (boundary == B_CONSTANT && outside) {
+            v = *cval;
+          } else if (boundary == B_FUNCTION && outside) {
+            const float fy = cval[0] + cval[1] * (float)yy;  /* in Float32, rounded per step */
+            v = (double)(fy + cval[2] * (float)xx);
+          } else {
+            v = in[bmap(boundary, yy, h) * wd + bmap(boundary, xx, wd)];
+          }
           const double p = (double)w[(dy + ry) * (2 * rx + 1) + (dx + rx)] * v;
           acc += p;
           ab += fabs(p);

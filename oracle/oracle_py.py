@@ -48,9 +48,10 @@ def _ptr(a):
 
 
 def _seed(dtype, seed):
+    """A scalar, or a sequence (the function boundary's c0, cy, cx), as a C array."""
     if seed is None:
         return None, None
-    a = np.array([seed], dtype=dtype)
+    a = np.ascontiguousarray(np.array(seed, dtype=dtype).reshape(-1))
     return a, _ptr(a)
 
 

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 ADD, MUL, MIN, MAX, AND, OR, XOR, SUB, EXCHANGE = range(9)
 L2R, R2L = 0, 1
-CLAMP, MIRROR, WRAP, CONSTANT = range(4)
+CLAMP, MIRROR, WRAP, CONSTANT, FUNCTION = range(5)
 FP_REL = 1e-5
 
 
@@ -568,6 +568,57 @@ def test_stencil_int_large_image(stream, r):
     for b in (CLAMP, MIRROR, WRAP, CONSTANT):
         A.stencil2d(stream, ry, rx, wt, b, np.int32(7), d, h, w, out)
         assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(ry, rx, wt, b, np.int32(7), img)), b
+
+
+# ---------------------------------------------------- function boundary
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+@pytest.mark.parametrize("r", [(1, 1), (2, 3), (4, 4)])
+@pytest.mark.parametrize("shape", [(130, 211), (300, 512), (5, 7)])
+def test_stencil_function_boundary_int(stream, shape, r, dt):
+    # an index outside reads (c0 + cy*y) + cx*x; 512-wide rows take the
+    # vector-staged interior tiles, the frame tiles evaluate f
+    ry, rx = r
+    h, w = shape
+    rng = np.random.default_rng(h * 3 + ry)
+    img = rng.integers(-1000, 1000, size=(h, w)).astype(dt)
+    wt = rng.integers(-3, 4, size=(2 * ry + 1, 2 * rx + 1)).astype(dt)
+    out = A.DeviceArray(h * w, dt)
+    c = (5, -3, 7)
+    A.stencil2d(stream, ry, rx, wt, FUNCTION, c, dev(stream, img), h, w, out)
+    assert np.array_equal(out.to_numpy(stream).reshape(h, w), O.stencil2d(ry, rx, wt, FUNCTION, c, img))
+
+
+@pytest.mark.parametrize("r", [(1, 1), (2, 2)])
+@pytest.mark.parametrize("shape", [(131, 515), (256, 1024)])
+def test_stencil_function_boundary_float32(stream, shape, r):
+    ry, rx = r
+    h, w = shape
+    img = O.gen_normal_f32(h + w, h * w).reshape(h, w)
+    wt = np.random.default_rng(ry).standard_normal((2 * ry + 1, 2 * rx + 1)).astype(np.float32)
+    c = (0.5, 0.25, -0.125)
+    out = A.DeviceArray(h * w, np.float32)
+    A.stencil2d(stream, ry, rx, wt, FUNCTION, c, dev(stream, img), h, w, out)
+    got = out.to_numpy(stream).reshape(h, w).astype(np.float64)
+    truth, ab = O.stencil2d_f64(ry, rx, wt, FUNCTION, c, img)
+    taps = (2 * ry + 1) * (2 * rx + 1)
+    assert np.all(np.abs(got - truth) <= taps * 2.0 ** -23 * ab + 1e-30)
+
+
+def test_stencil2_function_boundary_and_3d_refusal(stream):
+    rng = np.random.default_rng(31)
+    a = rng.integers(-1000, 1000, size=(70, 130)).astype(np.int32)
+    b = rng.integers(-1000, 1000, size=(66, 140)).astype(np.int32)
+    wa = rng.integers(-3, 4, size=(5, 3)).astype(np.int32)
+    wb = rng.integers(-3, 4, size=(5, 3)).astype(np.int32)
+    out = A.DeviceArray(66 * 130, np.int32)
+    A.stencil2_2d(stream, 2, 1, wa, FUNCTION, (1, 2, 3), dev(stream, a), 70, 130, wb, FUNCTION, (-4, 5, -6),
+                  dev(stream, b), 66, 140, out)
+    want = O.stencil2_2d(2, 1, wa, FUNCTION, (1, 2, 3), a, wb, FUNCTION, (-4, 5, -6), b)
+    assert np.array_equal(out.to_numpy(stream).reshape(66, 130), want)
+    vol = A.DeviceArray(27, np.float32)
+    with pytest.raises(A.AccError):
+        A.stencil3d(stream, 1, 1, 1, np.ones(27, np.float32), FUNCTION, (1.0, 0.0, 0.0), vol, 3, 3, 3,
+                    A.DeviceArray(27, np.float32))
 
 
 # ------------------------------------------------------------- stencil2

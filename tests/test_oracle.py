@@ -17,7 +17,7 @@ import oracle_py as O
 
 ADD, MUL, MIN, MAX, AND, OR, XOR, SUB, EXCHANGE = range(9)
 L2R, R2L = 0, 1
-CLAMP, MIRROR, WRAP, CONSTANT = range(4)
+CLAMP, MIRROR, WRAP, CONSTANT, FUNCTION = range(5)
 
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_examples.json")))
 
@@ -257,6 +257,55 @@ def test_integer_stencil_exact():
     out = O.stencil2d(1, 1, w, MIRROR, None, img)
     want = _scipy_correlate(img, w, MIRROR).astype(np.int64)
     assert np.array_equal(out.astype(np.int64), want)
+
+
+def test_function_boundary_by_hand_and_reductions():
+    # an index outside reads (c0 + cy*y) + cx*x of the index itself
+    img = np.arange(5, dtype=np.float64).reshape(1, 5)
+    c = (10.0, 2.0, 3.0)
+    left = np.zeros((3, 3))
+    left[1, 0] = 1.0   # (y, x - 1)
+    up = np.zeros((3, 3))
+    up[0, 1] = 1.0     # (y - 1, x)
+    corner = np.zeros((3, 3))
+    corner[0, 0] = 1.0  # (y - 1, x - 1)
+    assert O.stencil2d(1, 1, left, FUNCTION, c, img)[0, 0] == 10.0 + 3.0 * -1   # (0, -1)
+    assert O.stencil2d(1, 1, left, FUNCTION, c, img)[0, 3] == 2.0               # inside: x[0, 2]
+    assert O.stencil2d(1, 1, up, FUNCTION, c, img)[0, 2] == 10.0 + 2.0 * -1 + 3.0 * 2
+    assert O.stencil2d(1, 1, corner, FUNCTION, c, img)[0, 0] == 10.0 - 2.0 - 3.0
+    # cy = cx = 0 is the constant boundary
+    rng = np.random.default_rng(4)
+    im = rng.integers(-1000, 1000, size=(19, 23)).astype(np.int32)
+    w = rng.integers(-3, 4, size=(5, 3)).astype(np.int32)
+    assert np.array_equal(O.stencil2d(2, 1, w, FUNCTION, (7, 0, 0), im), O.stencil2d(2, 1, w, CONSTANT, 7, im))
+
+
+def test_function_boundary_against_padded_correlate():
+    # the image padded with f at the outside indices, correlated by scipy
+    ndimage = pytest.importorskip("scipy.ndimage")
+    rng = np.random.default_rng(12)
+    ry, rx, h, w = 2, 3, 17, 29
+    im = rng.integers(-1000, 1000, size=(h, w)).astype(np.int64)
+    wt = rng.integers(-3, 4, size=(2 * ry + 1, 2 * rx + 1)).astype(np.int64)
+    c0, cy, cx = 5, -3, 7
+    ys = np.arange(-ry, h + ry)[:, None]
+    xs = np.arange(-rx, w + rx)[None, :]
+    pad = (c0 + cy * ys + cx * xs).astype(np.float64)
+    pad[ry:ry + h, rx:rx + w] = im
+    want = ndimage.correlate(pad, wt.astype(np.float64), mode="constant")[ry:ry + h, rx:rx + w]
+    got = O.stencil2d(ry, rx, wt, FUNCTION, (c0, cy, cx), im)
+    assert np.array_equal(got.astype(np.float64), want)
+
+
+def test_function_boundary_integer_wraps():
+    # Int32: (c0 + cy*y) + cx*x wraps modulo 2^32
+    img = np.zeros((2, 2), np.int32)
+    up_left = np.zeros((3, 3), np.int32)
+    up_left[0, 0] = 1  # (y - 1, x - 1) = (-1, -1) at the origin
+    c = (2**31 - 5, 2**31 - 1, -(2**31))
+    got = int(O.stencil2d(1, 1, up_left, FUNCTION, c, img)[0, 0])
+    want = (c[0] + c[1] * -1 + c[2] * -1) % 2**32
+    assert got % 2**32 == want
 
 
 def test_stencil2_against_scipy_and_linearity():

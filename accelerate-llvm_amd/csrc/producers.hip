@@ -301,10 +301,12 @@ __global__ __launch_bounds__(PB) void k_gather1(const T* in, int64_t n_in, const
   }
 }
 
-// reverse: out[i] = f(in[n - 1 - i]); output chunks of 16 bytes, the reads
-// element by element (a wave's reads are still one contiguous, descending
-// range).
-template <typename T, typename F>
+// reverse: out[i] = f(in[n - 1 - i]); output chunks of 16 bytes.  VIN (the
+// input 16-byte aligned and n a multiple of the chunk): each chunk's source
+// is one aligned 16-byte block, read whole and reversed in registers;
+// otherwise the reads go element by element (a wave's reads are still one
+// contiguous, descending range).
+template <typename T, typename F, bool VIN>
 __global__ __launch_bounds__(PB) void k_reverse(const T* in, T* out, int64_t n, F f, int aligned) {
   constexpr int E = 16 / (int)sizeof(T);
   const int64_t stride = (int64_t)gridDim.x * PB;
@@ -318,8 +320,14 @@ __global__ __launch_bounds__(PB) void k_reverse(const T* in, T* out, int64_t n, 
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
       const int64_t c = min(c0 + u * PB, nchunk - 1);
+      if (VIN) {
+        const Vec<T> v = load16_stream(in + (n - (c + 1) * E));  // elements n-(c+1)E .. n-1-cE
 #pragma unroll
-      for (int j = 0; j < E; ++j) x[u][j] = in[n - 1 - (c * E + j)];
+        for (int j = 0; j < E; ++j) x[u][j] = v.e[E - 1 - j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < E; ++j) x[u][j] = in[n - 1 - (c * E + j)];
+      }
     }
 #pragma unroll
     for (int u = 0; u < PU; ++u) {
@@ -580,8 +588,13 @@ extern "C" int acc_hip_transform(acc_stream_t s, acc_type t, acc_ixmap p, acc_un
         return fail(ACC_ERROR_UNSUPPORTED, "backpermute: index type %d", (int)index_type);
       }
       if (p == ACC_IX_REVERSE) {
-        hipLaunchKernelGGL((k_reverse<T, F>), dim3(stream_grid(n, cus)), dim3(PB), 0, s->stream, (const T*)in,
-                           (T*)out, n, fn, al16(out) ? 1 : 0);
+        constexpr int E = 16 / (int)sizeof(T);
+        if (al16(out) && al16(in) && n % E == 0)
+          hipLaunchKernelGGL((k_reverse<T, F, true>), dim3(stream_grid(n, cus)), dim3(PB), 0, s->stream,
+                             (const T*)in, (T*)out, n, fn, 1);
+        else
+          hipLaunchKernelGGL((k_reverse<T, F, false>), dim3(stream_grid(n, cus)), dim3(PB), 0, s->stream,
+                             (const T*)in, (T*)out, n, fn, al16(out) ? 1 : 0);
         ACC_HIP_CHECK(hipGetLastError());
         return ACC_SUCCESS;
       }

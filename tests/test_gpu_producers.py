@@ -169,13 +169,20 @@ def test_gather_empty_source(stream):
     same(o.to_numpy(stream), np.zeros(10, np.int64))
 
 
-@pytest.mark.parametrize("n", [1, 4, 5, 1027, (1 << 20) + 7])
+@pytest.mark.parametrize("n", [1, 4, 5, 1027, 4096, 1 << 20, (1 << 20) + 7])
 @pytest.mark.parametrize("dt", TYPES)
 def test_reverse(stream, dt, n):
+    # n a multiple of the 16-byte chunk with aligned arrays: whole-vector reads
     x = vals(dt, n, n)
     o = A.DeviceArray(n, dt)
     A.transform(stream, A.IX_REVERSE, A.UNOP_ABS, dev(stream, x), 1, n, None, n, o)
     same(o.to_numpy(stream), O.transform(O.IX_REVERSE, O.UNOP_ABS, x, 1, n))
+    if n >= 4:
+        # the input one element in (not 16-byte aligned): element reads
+        d = dev(stream, vals(dt, n + 1, n + 1))
+        xs = d.to_numpy(stream)[1:]
+        A.transform(stream, A.IX_REVERSE, A.UNOP_ID, d.view(1, n), 1, n, None, n, o)
+        same(o.to_numpy(stream), O.transform(O.IX_REVERSE, O.UNOP_ID, xs, 1, n))
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (1, 100), (100, 1), (63, 65), (64, 64), (129, 257), (1000, 999),

@@ -164,11 +164,32 @@ int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_
 }
 
 // ---------------------------------------------------------------- fill kernel
+// vec (dst aligned to the element): a head of elements up to the first
+// 16-byte boundary, whole 16-byte stores of the replicated value (ordinary
+// stores: for write-only streams they measured faster than non-temporal
+// ones), then a tail of elements; otherwise element by element.
 template <typename W>
-__global__ void k_fill(W* __restrict__ dst, W value, size_t count) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < count; i += stride) dst[i] = value;
+__global__ __launch_bounds__(256) void k_fill(W* __restrict__ dst, W value, size_t count, int vec) {
+  constexpr int E = 16 / (int)sizeof(W);
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  if (!vec) {
+    for (size_t i = tid; i < count; i += stride) dst[i] = value;
+    return;
+  }
+  const size_t mis = ((uintptr_t)dst % 16) / sizeof(W);
+  const size_t head = mis ? min((size_t)E - mis, count) : 0;
+  if (tid < head) dst[tid] = value;
+  W rep[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) rep[j] = value;
+  u32x4 v;
+  __builtin_memcpy(&v, rep, 16);
+  u32x4* vd = (u32x4*)(dst + head);
+  const size_t nv = (count - head) / E;
+  for (size_t i = tid; i < nv; i += stride) vd[i] = v;
+  const size_t done = head + nv * E;  // count - done < E
+  if (tid < count - done) dst[done + tid] = value;
 }
 
 // Device-to-device copy of nv 16-byte vectors: four non-temporal dwordx4
@@ -324,14 +345,17 @@ int acc_hip_fill_async(acc_stream_t s, void* dst, const void* value, size_t elem
   if (!count) return ACC_SUCCESS;
   if (!dst) return fail(ACC_ERROR_INVALID_VALUE, "null destination");
   ACC_CHECK(set_device(s->device));
-  unsigned grid = (unsigned)((count + 255) / 256);
-  if (grid > 8192) grid = 8192;
+  if (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+    return fail(ACC_ERROR_INVALID_VALUE, "fill element size %zu", elem_bytes);
+  const int vec = (uintptr_t)dst % elem_bytes == 0 ? 1 : 0;
+  // one thread per 16 bytes (per element without vec), at most 8192 workgroups
+  const size_t work = vec ? count * elem_bytes / 16 + 1 : count;
+  const unsigned grid = (unsigned)std::min<size_t>((work + 255) / 256, 8192);
   switch (elem_bytes) {
-    case 1: { uint8_t v; memcpy(&v, value, 1); hipLaunchKernelGGL(k_fill<uint8_t>, grid, 256, 0, s->stream, (uint8_t*)dst, v, count); break; }
-    case 2: { uint16_t v; memcpy(&v, value, 2); hipLaunchKernelGGL(k_fill<uint16_t>, grid, 256, 0, s->stream, (uint16_t*)dst, v, count); break; }
-    case 4: { uint32_t v; memcpy(&v, value, 4); hipLaunchKernelGGL(k_fill<uint32_t>, grid, 256, 0, s->stream, (uint32_t*)dst, v, count); break; }
-    case 8: { uint64_t v; memcpy(&v, value, 8); hipLaunchKernelGGL(k_fill<uint64_t>, grid, 256, 0, s->stream, (uint64_t*)dst, v, count); break; }
-    default: return fail(ACC_ERROR_INVALID_VALUE, "fill element size %zu", elem_bytes);
+    case 1: { uint8_t v; memcpy(&v, value, 1); hipLaunchKernelGGL(k_fill<uint8_t>, grid, 256, 0, s->stream, (uint8_t*)dst, v, count, vec); break; }
+    case 2: { uint16_t v; memcpy(&v, value, 2); hipLaunchKernelGGL(k_fill<uint16_t>, grid, 256, 0, s->stream, (uint16_t*)dst, v, count, vec); break; }
+    case 4: { uint32_t v; memcpy(&v, value, 4); hipLaunchKernelGGL(k_fill<uint32_t>, grid, 256, 0, s->stream, (uint32_t*)dst, v, count, vec); break; }
+    case 8: { uint64_t v; memcpy(&v, value, 8); hipLaunchKernelGGL(k_fill<uint64_t>, grid, 256, 0, s->stream, (uint64_t*)dst, v, count, vec); break; }
   }
   ACC_HIP_CHECK(hipGetLastError());
   return ACC_SUCCESS;

@@ -216,6 +216,22 @@ def test_transpose_views_and_identity(stream):
     same(t.to_numpy(stream), y.reshape(r, c).T.reshape(-1))
 
 
+@pytest.mark.parametrize("esize", [1, 2, 4, 8])
+def test_fill_async(stream, esize):
+    # element heads up to a 16-byte boundary, 16-byte stores, element tails:
+    # exactly [off, off + count) changes
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[esize]
+    v = np.array([0xA5C3_5A3C_9616_6996 & ((1 << (8 * esize)) - 1)], dtype=dt)
+    n = 5000
+    for off, cnt in ((0, 0), (0, 1), (1, 3), (3, 17), (0, 4096), (5, 4000), (7, 4993), (2, 1)):
+        d = dev(stream, np.zeros(n, dt))
+        A.check(A.lib().acc_hip_fill_async(stream.handle, A.c_void_p(d.ptr + off * esize),
+                                           v.ctypes.data_as(A.c_void_p), esize, cnt))
+        want = np.zeros(n, dt)
+        want[off:off + cnt] = v[0]
+        same(d.to_numpy(stream), want)
+
+
 def test_producer_errors(stream):
     d = A.DeviceArray(16, np.int32)
     f = A.DeviceArray(16, np.float32)

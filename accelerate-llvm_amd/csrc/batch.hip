@@ -330,6 +330,9 @@ static int batch_autotune(acc_batch_t b) {
   const std::string sig = batch_signature(b);
   int k = tune_lookup(sig);
   if (k == 1 || k == 2 || k == 4) return batch_layout(b, k);
+  // the timing runs read the inputs and write the outputs: order them after
+  // everything already queued on the device (inputs produced on any stream)
+  ACC_HIP_CHECK(hipDeviceSynchronize());
   hipStream_t st;
   ACC_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   hipEvent_t e0, e1;

@@ -5,6 +5,7 @@
 #include <string>
 #include <mutex>
 #include <vector>
+#include <memory>
 #include "../../include/accelerate_hip.h"
 
 namespace acc {
@@ -43,19 +44,32 @@ int dev_free(int device, void* p);
 
 }  // namespace acc
 
-// A stream and the device workspace that its skeletons use.  The workspace is
-// one allocation: a SyncCtl block (tickets, epoch) followed by tile-status
-// words.  It only grows, and growth synchronises the stream first.
+namespace acc {
+// One device allocation of skeleton workspace: [SyncCtl (256 B) | status
+// words].  Shared ownership: the stream that allocated it and every graph
+// captured on that stream while it was current keep it alive, since a graph's
+// kernels hold its address as frozen arguments.
+struct Workspace {
+  int device = 0;
+  void* ptr = nullptr;
+  size_t status_bytes = 0;  // capacity of the status region
+  ~Workspace();
+};
+}  // namespace acc
+
+// A stream and the device workspace that its skeletons use.  The workspace
+// only grows; growth allocates a new one and retires the old one (which
+// queued work may still use) without synchronising.
 struct acc_stream_s {
   int device = 0;
+  uint64_t id = 0;  // unique per stream created by the library (never reused)
   hipStream_t stream = nullptr;
   std::mutex mu;
-  void* ws = nullptr;          // device: [SyncCtl (256 B) | status words]
-  size_t ws_status_bytes = 0;  // capacity of the status region
+  std::shared_ptr<acc::Workspace> ws;
   bool capturing = false;
-  // workspaces outgrown by this stream: freed when the stream is destroyed
+  // workspaces outgrown by this stream: released when the stream is destroyed
   // (hipFree would synchronise the whole device at the moment of growth)
-  std::vector<void*> retired;
+  std::vector<std::shared_ptr<acc::Workspace>> retired;
 };
 
 struct acc_event_s {
@@ -70,8 +84,10 @@ struct acc_module_s {
 
 struct acc_graph_s {
   int device = 0;
+  uint64_t stream_id = 0;  // the capture stream: the only stream it may be launched on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::shared_ptr<acc::Workspace> ws;  // the capture stream's workspace, kept alive
 };
 
 namespace acc {

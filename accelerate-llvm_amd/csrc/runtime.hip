@@ -9,6 +9,7 @@
 #include <unordered_map>
 #include <vector>
 #include <memory>
+#include <atomic>
 #include "host.hpp"
 #include "vecmem.hpp"
 #include "lookback.hpp"
@@ -28,6 +29,9 @@ int fail(int code, const char* fmt, ...) {
 }
 
 int hip_fail(hipError_t e, const char* what) {
+  // the runtime keeps the error as the thread's "last error": clear it, so
+  // that the launch check of a later call does not report this failure again
+  (void)hipGetLastError();
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
     return fail(ACC_ERROR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
   if (e == hipErrorNoDevice) return fail(ACC_ERROR_NO_DEVICE, "%s: %s", what, hipGetErrorString(e));
@@ -138,28 +142,35 @@ int dev_free(int device, void* p) {
 // ------------------------------------------------------------------ streams
 static constexpr size_t kCtlBytes = 256;
 
+Workspace::~Workspace() {
+  if (ptr) (void)dev_free(device, ptr);
+}
+
 int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status) {
   if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
   ACC_CHECK(set_device(s->device));
   std::lock_guard<std::mutex> lk(s->mu);
-  if (!s->ws || status_bytes > s->ws_status_bytes) {
+  if (!s->ws || status_bytes > s->ws->status_bytes) {
     if (s->capturing)
       return fail(ACC_ERROR_INVALID_VALUE,
                   "stream workspace must grow to %zu bytes during capture; call "
                   "acc_hip_stream_reserve before capturing", status_bytes);
-    size_t cap = s->ws_status_bytes ? s->ws_status_bytes : (size_t)1 << 20;
+    size_t cap = s->ws ? s->ws->status_bytes : (size_t)1 << 20;
     while (cap < status_bytes) cap *= 2;
-    void* ws = nullptr;
-    ACC_CHECK(dev_malloc(s->device, kCtlBytes + cap, &ws));
-    ACC_HIP_CHECK(hipMemsetAsync(ws, 0, kCtlBytes + cap, s->stream));
+    auto ws = std::make_shared<Workspace>();
+    ws->device = s->device;
+    ACC_CHECK(dev_malloc(s->device, kCtlBytes + cap, &ws->ptr));
+    ws->status_bytes = cap;
+    // not under capture (refused above), so the clear is an ordinary
+    // stream-ordered memset
+    ACC_HIP_CHECK(hipMemsetAsync(ws->ptr, 0, kCtlBytes + cap, s->stream));
     // the old workspace may still be in use by queued work: it is retired,
     // not freed (no synchronisation of the stream or the device here)
     if (s->ws) s->retired.push_back(s->ws);
     s->ws = ws;
-    s->ws_status_bytes = cap;
   }
-  *ctl = (SyncCtl*)s->ws;
-  *status = (uint64_t*)((char*)s->ws + kCtlBytes);
+  *ctl = (SyncCtl*)s->ws->ptr;
+  *status = (uint64_t*)((char*)s->ws->ptr + kCtlBytes);
   return ACC_SUCCESS;
 }
 
@@ -366,8 +377,10 @@ int acc_hip_stream_create(int device, acc_stream_t* out) {
   const DeviceInfo* di;
   ACC_CHECK(device_info(device, &di));
   ACC_CHECK(set_device(device));
+  static std::atomic<uint64_t> next_id{1};
   auto s = std::make_unique<acc_stream_s>();
   s->device = device;
+  s->id = next_id++;
   ACC_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   *out = s.release();
   return ACC_SUCCESS;
@@ -378,8 +391,9 @@ int acc_hip_stream_destroy(acc_stream_t s) {
   int rc = set_device(s->device);
   if (rc == ACC_SUCCESS) {
     (void)hipStreamSynchronize(s->stream);
-    if (s->ws) dev_free(s->device, s->ws);
-    for (void* p : s->retired) dev_free(s->device, p);
+    // a workspace that a live graph still references outlives the stream
+    s->ws.reset();
+    s->retired.clear();
     (void)hipStreamDestroy(s->stream);
   }
   delete s;
@@ -529,6 +543,11 @@ int acc_hip_graph_end_capture(acc_stream_t s, acc_graph_t* out) {
   }
   auto g = std::make_unique<acc_graph_s>();
   g->device = s->device;
+  g->stream_id = s->id;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    g->ws = s->ws;  // growth is refused during capture: this is the one the kernels captured
+  }
   ACC_HIP_CHECK(hipStreamEndCapture(s->stream, &g->graph));
   hipError_t e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
@@ -541,6 +560,11 @@ int acc_hip_graph_end_capture(acc_stream_t s, acc_graph_t* out) {
 
 int acc_hip_graph_launch(acc_graph_t g, acc_stream_t s) {
   if (!g || !s) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  // the graph's skeleton kernels use the capture stream's tickets and status
+  // words: a replay on another stream could interleave with that stream's own
+  // skeletons on the same counters
+  if (s->id != g->stream_id)
+    return fail(ACC_ERROR_INVALID_VALUE, "graph: launch on a stream other than the one it was captured on");
   ACC_CHECK(set_device(s->device));
   ACC_HIP_CHECK(hipGraphLaunch(g->exec, s->stream));
   return ACC_SUCCESS;

@@ -110,7 +110,7 @@ struct acc_server_s {
   std::vector<uint64_t> slot_req;  // 0 = idle
   std::mutex mu;
   std::condition_variable cv;
-  std::map<uint64_t, std::unique_ptr<Req>> reqs;
+  std::map<uint64_t, std::shared_ptr<Req>> reqs;  // a waiter holds its own reference
   std::deque<uint64_t> pending;  // arrival order (by arrival time)
   std::vector<double> latencies_us;
   clk::time_point first_arrival{}, last_done{};
@@ -188,14 +188,14 @@ static void slot_worker(acc_server_t s, int slot) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     const auto now = clk::now();
-    Req* pick = nullptr;
+    std::shared_ptr<Req> pick;
     size_t off = 0;
     for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
-      Req& q = *s->reqs[*it];
-      if (q.arrival > now) break;  // pending is ordered by arrival
-      if (!s->arena.alloc(q.out_bytes, &off)) continue;
+      const std::shared_ptr<Req>& q = s->reqs[*it];
+      if (q->arrival > now) break;  // pending is ordered by arrival
+      if (!s->arena.alloc(q->out_bytes, &off)) continue;
+      pick = q;
       s->pending.erase(it);
-      pick = &q;
       break;
     }
     if (pick) {
@@ -220,6 +220,10 @@ static void slot_worker(acc_server_t s, int slot) {
       if (rc) {
         q.status = rc;
         s->failed++;
+        // kernels queued before the failure may still write the range
+        lk.unlock();
+        (void)hipStreamSynchronize(st->stream);
+        lk.lock();
         s->arena.release(off, q.out_bytes);
       } else {
         s->completed++;
@@ -290,7 +294,7 @@ int acc_hip_server_submit(acc_server_t s, const acc_request* r, int64_t arrival_
   if (r->skeleton < ACC_REQ_FOLD || r->skeleton > ACC_REQ_STENCIL2D)
     return fail(ACC_ERROR_UNSUPPORTED, "server: request kind %d", r->skeleton);
   if (!type_size((acc_type)r->type)) return fail(ACC_ERROR_UNSUPPORTED, "server: element type %d", r->type);
-  auto q = std::make_unique<Req>();
+  auto q = std::make_shared<Req>();
   q->r = *r;
   q->keep = keep_output != 0;
   q->out_bytes = request_out_bytes(*r);
@@ -323,10 +327,13 @@ int acc_hip_server_wait(acc_server_t s, uint64_t id, void** out, double* latency
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return fail(ACC_ERROR_INVALID_VALUE, "server: unknown or released request %llu",
                                        (unsigned long long)id);
-  Req* q = it->second.get();
+  // the waiter's own reference keeps the record alive when the worker
+  // retires a request submitted with keep_output == 0
+  std::shared_ptr<Req> q = it->second;
   s->cv.wait(lk, [&] { return q->finished; });
   if (q->status) return fail(q->status, "server: request %llu failed", (unsigned long long)id);
-  if (out) *out = (char*)s->arena_base + q->out_off;
+  // a result that was not kept is reused once complete: no pointer to it
+  if (out) *out = q->keep ? (char*)s->arena_base + q->out_off : nullptr;
   if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(q->done - q->arrival).count();
   return ACC_SUCCESS;
 }

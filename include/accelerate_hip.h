@@ -184,7 +184,11 @@ int acc_hip_launch(acc_function_t f, unsigned gx, unsigned gy, unsigned gz, unsi
                    void** args);
 
 /* ----------------------------------------------------------------- graphs
- * Stream capture into a hipGraph (new: the batched form). */
+ * Stream capture into a hipGraph (new: the batched form).  A graph captured on
+ * stream s replays only on s (launch on another stream is
+ * ACC_ERROR_INVALID_VALUE): its skeleton kernels use s's tickets and status
+ * words.  The graph keeps that workspace alive until acc_hip_graph_destroy,
+ * also when s is destroyed first. */
 int acc_hip_graph_begin_capture(acc_stream_t s);
 int acc_hip_graph_end_capture(acc_stream_t s, acc_graph_t* g);
 int acc_hip_graph_launch(acc_graph_t g, acc_stream_t s);
@@ -364,7 +368,11 @@ typedef struct acc_batch_s* acc_batch_t;
 
 /* tile = vectors per thread (1, 2 or 4); 0 lets the autotuner time the
  * candidates on this batch and remember the winner (file named by
- * $ACC_HIP_TUNE_CACHE, else ~/.cache/accelerate-hip-tune.txt). */
+ * $ACC_HIP_TUNE_CACHE, else ~/.cache/accelerate-hip-tune.txt).  Tuning RUNS
+ * the batch several times (a cached choice does not): it first waits for all
+ * work queued on the device, so inputs written by earlier calls on any stream
+ * are complete, and it writes every problem's output, so the outputs must not
+ * be in use by other work while acc_hip_batch_create runs. */
 int acc_hip_batch_create(int device, const acc_batch_problem* problems, int64_t count, int tile,
                          acc_batch_t* b);
 int acc_hip_batch_launch(acc_batch_t b, acc_stream_t s);
@@ -420,7 +428,11 @@ typedef struct acc_server_s* acc_server_t;
 /* hbm_budget: arena bytes (0 = half of the free device memory) */
 int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t* s);
 /* arrival = now + arrival_offset_ns; keep_output: hold the result until
- * acc_hip_server_release (otherwise its memory is reused on completion) */
+ * acc_hip_server_release (otherwise its memory is reused on completion).
+ * acc_hip_server_wait blocks until the request is complete.  For a kept
+ * request it returns the output's device address; for one submitted with
+ * keep_output == 0 it returns *out = NULL, and once such a request has
+ * completed and been retired its id is unknown (ACC_ERROR_INVALID_VALUE). */
 int acc_hip_server_submit(acc_server_t s, const acc_request* r, int64_t arrival_offset_ns, int keep_output,
                           uint64_t* id);
 int acc_hip_server_wait(acc_server_t s, uint64_t id, void** out, double* latency_us);

@@ -137,3 +137,22 @@ def test_server_future_arrivals_and_drain(stream, pool):
     st = srv.stats()
     assert st.completed == 20 and st.span_s >= 0.0038
     srv.destroy()
+
+
+def test_server_wait_on_a_request_that_is_not_kept(stream, pool):
+    """A waiter on a keep=False request must not touch the record the worker
+    retires on completion: it returns success and no output address."""
+    srv = A.Server(0, slots=2)
+    x = pool["i32"]
+    ids = [srv.submit(A.Server.request(A.REQ_SCAN, np.int32, A.ADD, src=x, n=(1 << 20) - k)) for k in range(8)]
+    for rid in ids:
+        try:
+            ptr, lat = srv.wait(rid)
+        except A.AccError as e:
+            # already completed and retired before the wait: the id is unknown
+            assert e.status == A.INVALID_VALUE
+            continue
+        assert ptr is None and lat > 0
+    srv.drain()
+    assert srv.stats().completed == 8
+    srv.destroy()

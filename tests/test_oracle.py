@@ -62,6 +62,19 @@ def test_fold_add_wraps_like_numpy(dt):
     assert O.fold(OR, None, x)[0] == np.bitwise_or.reduce(x)
 
 
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint32, np.uint64])
+def test_fold_mul_wraps_like_numpy(dt):
+    # odd factors: their wrapped product is odd, so it cannot collapse to 0
+    # (random factors make the product 0 mod 2^k after a few dozen terms)
+    rng = np.random.default_rng(2)
+    info = np.iinfo(dt)
+    x = (rng.integers(info.min // 2, info.max // 2, size=4099, dtype=dt, endpoint=True) | dt(1)).astype(dt)
+    want = np.multiply.reduce(x, dtype=dt)
+    assert want % 2 == 1
+    assert O.fold(MUL, None, x)[0] == want
+    assert O.fold(MUL, dt(3), x)[0] == np.multiply.reduce(np.concatenate([np.array([3], dt), x]), dtype=dt)
+
+
 def test_fold_rows_and_empty():
     x = np.arange(12, dtype=np.int32).reshape(3, 4)
     assert O.fold(ADD, 100, x, outer=3).tolist() == [106, 122, 138]

@@ -176,6 +176,8 @@ SIGNATURES = {
     "acc_hip_fold": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     "acc_hip_fold_seg": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_int, c_void_p,
                                  c_int64, c_void_p]),
+    "acc_hip_fold_seg_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_int,
+                                         c_void_p, c_int64, c_void_p]),
     "acc_hip_scan": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     "acc_hip_scan_prime": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
                                    c_void_p]),
@@ -385,6 +387,13 @@ def fold_seg(stream, op, seed, src, outer, inner, offsets, nseg, out):
     keep, sp = _scalar(src.dtype, seed)
     check(lib().acc_hip_fold_seg(stream.handle, acc_type(src.dtype), op, sp, _p(src), outer, inner,
                                  acc_type(offsets.dtype), _p(offsets), nseg, _p(out)))
+
+
+def fold_seg_noalloc(stream, op, seed, src, outer, inner, offsets, nseg, out):
+    """fold_seg with the library's fixed device scratch instead of allocated workspace."""
+    keep, sp = _scalar(src.dtype, seed)
+    check(lib().acc_hip_fold_seg_noalloc(stream.handle, acc_type(src.dtype), op, sp, _p(src), outer, inner,
+                                         acc_type(offsets.dtype), _p(offsets), nseg, _p(out)))
 
 
 def scan(stream, op, direction, seed, src, outer, inner, out):

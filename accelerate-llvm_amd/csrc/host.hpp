@@ -93,8 +93,20 @@ struct acc_graph_s {
 namespace acc {
 
 // Workspace of `s` with at least `status_bytes` of status words; sets the
-// device.  Returns the SyncCtl pointer and the status region.
+// device.  Returns the SyncCtl pointer and the status region.  While a
+// ScratchOverride is alive on the calling thread, the override's memory is
+// returned instead (nothing is allocated) or, if it is too small, an error.
 int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status);
+
+// Routes the skeleton workspace of the calling thread's launches to a
+// caller-provided device region [SyncCtl (256 B) | status words] for the
+// lifetime of the object (the allocation-free entry points).
+struct ScratchOverride {
+  ScratchOverride(int device, void* base, size_t bytes);
+  ~ScratchOverride();
+  ScratchOverride(const ScratchOverride&) = delete;
+  ScratchOverride& operator=(const ScratchOverride&) = delete;
+};
 
 inline int set_device(int device) {
   hipError_t e = hipSetDevice(device);

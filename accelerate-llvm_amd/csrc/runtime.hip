@@ -146,9 +146,30 @@ Workspace::~Workspace() {
   if (ptr) (void)dev_free(device, ptr);
 }
 
+struct ScratchRegion {
+  int device = -1;
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+static thread_local ScratchRegion tl_scratch;
+
+ScratchOverride::ScratchOverride(int device, void* base, size_t bytes) { tl_scratch = {device, base, bytes}; }
+ScratchOverride::~ScratchOverride() { tl_scratch = ScratchRegion{}; }
+
 int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status) {
   if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
   ACC_CHECK(set_device(s->device));
+  if (tl_scratch.base) {
+    if (tl_scratch.device != s->device)
+      return fail(ACC_ERROR_INVALID_VALUE, "scratch of device %d used on device %d", tl_scratch.device, s->device);
+    if (kCtlBytes + status_bytes > tl_scratch.bytes)
+      return fail(ACC_ERROR_INVALID_VALUE,
+                  "allocation-free call needs %zu bytes of scratch; the maximum is %zu bytes",
+                  kCtlBytes + status_bytes, tl_scratch.bytes);
+    *ctl = (SyncCtl*)tl_scratch.base;
+    *status = (uint64_t*)((char*)tl_scratch.base + kCtlBytes);
+    return ACC_SUCCESS;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   if (!s->ws || status_bytes > s->ws->status_bytes) {
     if (s->capturing)

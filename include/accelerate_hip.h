@@ -219,6 +219,21 @@ int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const void* seed, co
                      int64_t outer, int64_t inner, acc_type offset_type, const void* offsets,
                      int64_t nseg, void* out);
 
+/* Allocation-free foldSeg / fold1Seg.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and kernels as acc_hip_fold_seg, but the skeleton workspace is a
+ * fixed 16 MiB device array inside the library (one per device, held until
+ * the process ends) instead of the stream's allocated workspace, so the call
+ * allocates and frees nothing.  A call whose workspace would exceed 16 MiB
+ * (about 2^28.9 Float32 or 2^27.9 Float64 elements per row) is refused with
+ * ACC_ERROR_INVALID_VALUE before anything is launched.  All calls on one
+ * device share that device's array: calls that overlap on one device (other
+ * streams, other threads) race and must be serialised by the caller. */
+int acc_hip_fold_seg_noalloc(acc_stream_t s, acc_type t, acc_op op, const void* seed, const void* in,
+                             int64_t outer, int64_t inner, acc_type offset_type, const void* offsets,
+                             int64_t nseg, void* out);
+
 /* scanl1/scanr1 (seed == NULL, out has inner elements per row) and
  * scanl/scanr (seed != NULL, out has inner + 1 elements per row)
  * (LLVM/Execute.hs:104-111; PTX/Execute.hs:377-488; element order

@@ -132,6 +132,34 @@ class ServerStats(ctypes.Structure):
         ("arena_used", c_size_t),
     ]
 
+class KernelInfo(ctypes.Structure):
+    _fields_ = [
+        ("origin", c_int),
+        ("lock_free_permute", c_int),
+        ("compile_ms", ctypes.c_double),
+        ("key", ctypes.c_char * 40),
+    ]
+
+
+class KernelArgs(ctypes.Structure):
+    _fields_ = [
+        ("in_", c_void_p * 4),
+        ("out", c_void_p * 4),
+        ("defaults", c_void_p * 4),
+        ("sums", c_void_p * 4),
+        ("offsets", c_void_p),
+        ("outer", ctypes.c_int64),
+        ("inner", ctypes.c_int64),
+        ("nseg", ctypes.c_int64),
+        ("nfree", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("free_ptr", c_void_p * 8),
+        ("free_len", ctypes.c_int64 * 8),
+    ]
+
+
+KERNEL_COMPILED, KERNEL_FROM_DISK_CACHE, KERNEL_FROM_IMAGE, KERNEL_LOADED = range(4)
+
 # exported symbol -> (restype, argtypes); every declaration in the header
 SIGNATURES = {
     "acc_hip_last_error": (c_char_p, []),
@@ -194,6 +222,14 @@ SIGNATURES = {
     "acc_hip_generate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p]),
     "acc_hip_transform": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_int,
                                   c_void_p, c_int64, c_void_p]),
+    "acc_hip_node_source": (c_int, [c_char_p, c_char_p, c_size_t, POINTER(c_size_t)]),
+    "acc_hip_node_precompile": (c_int, [c_char_p, POINTER(c_int)]),
+    "acc_hip_kernel_compile": (c_int, [c_int, c_char_p, POINTER(c_void_p)]),
+    "acc_hip_kernel_from_image": (c_int, [c_int, c_char_p, c_void_p, c_size_t, POINTER(c_void_p)]),
+    "acc_hip_kernel_image": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
+    "acc_hip_kernel_info": (c_int, [c_void_p, POINTER(KernelInfo)]),
+    "acc_hip_kernel_run": (c_int, [c_void_p, c_void_p, POINTER(KernelArgs)]),
+    "acc_hip_kernel_destroy": (c_int, [c_void_p]),
     "acc_hip_batch_create": (c_int, [c_int, POINTER(BatchProblem), c_int64, c_int, POINTER(c_void_p)]),
     "acc_hip_batch_launch": (c_int, [c_void_p, c_void_p]),
     "acc_hip_batch_tile": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_uint32)]),
@@ -627,3 +663,86 @@ def copy_to_host(stream, ptr, count, dtype):
         check(lib().acc_hip_memcpy_d2h_async(stream.handle, out.ctypes.data_as(c_void_p), c_void_p(ptr), out.nbytes))
         stream.synchronize()
     return out
+
+
+# ------------------------------------------------------ run-time compiled skeletons
+def node_source(node: str) -> str:
+    """The HIP source a skeleton node translates to."""
+    n = c_size_t()
+    check(lib().acc_hip_node_source(node.encode(), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    check(lib().acc_hip_node_source(node.encode(), buf, n.value + 1, ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def precompile(node: str) -> int:
+    """Compile a node into the on-disk code-object cache (no device needed)."""
+    o = c_int()
+    check(lib().acc_hip_node_precompile(node.encode(), ctypes.byref(o)))
+    return o.value
+
+
+def _ptrs(xs):
+    if xs is None:
+        return []
+    if not isinstance(xs, (list, tuple)):
+        xs = [xs]
+    return [x if isinstance(x, int) else (x.ptr if x is not None else 0) for x in xs]
+
+
+class Kernel:
+    """A skeleton node compiled at run time (acc_hip_kernel_*)."""
+
+    def __init__(self, node: str, device: int = 0, image: bytes = None):
+        h = c_void_p()
+        if image is None:
+            check(lib().acc_hip_kernel_compile(device, node.encode(), ctypes.byref(h)))
+        else:
+            buf = ctypes.create_string_buffer(image, len(image))
+            check(lib().acc_hip_kernel_from_image(device, node.encode(), buf, len(image), ctypes.byref(h)))
+        self.handle = h
+        self.node = node
+
+    def info(self) -> KernelInfo:
+        i = KernelInfo()
+        check(lib().acc_hip_kernel_info(self.handle, ctypes.byref(i)))
+        return i
+
+    def image(self) -> bytes:
+        n = c_size_t()
+        check(lib().acc_hip_kernel_image(self.handle, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(lib().acc_hip_kernel_image(self.handle, buf, n.value, ctypes.byref(n)))
+        return buf.raw[: n.value]
+
+    def run(self, stream, inputs=None, outputs=None, outer=1, inner=0, nseg=0, offsets=None, defaults=None,
+            sums=None, free=()):
+        """inputs/outputs/defaults/sums: a DeviceArray (one field) or a list (one per field);
+        free: sequence of DeviceArray (free arrays read by (read k i))."""
+        a = KernelArgs()
+        for k, p in enumerate(_ptrs(inputs)):
+            a.in_[k] = p
+        for k, p in enumerate(_ptrs(outputs)):
+            a.out[k] = p
+        for k, p in enumerate(_ptrs(defaults)):
+            a.defaults[k] = p
+        for k, p in enumerate(_ptrs(sums)):
+            a.sums[k] = p
+        a.offsets = offsets.ptr if offsets is not None else None
+        a.outer, a.inner, a.nseg = outer, inner, nseg
+        a.nfree = len(free)
+        for k, f in enumerate(free):
+            a.free_ptr[k] = f.ptr
+            a.free_len[k] = f.count
+        check(lib().acc_hip_kernel_run(self.handle, stream.handle, ctypes.byref(a)))
+
+    def destroy(self):
+        if self.handle:
+            lib().acc_hip_kernel_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,6 +1,8 @@
 // The stencil boundary index map, shared by the 2-D and 3-D stencils.
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 #include <type_traits>
 #include "../../include/accelerate_hip.h"

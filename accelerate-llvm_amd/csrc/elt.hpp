@@ -11,10 +11,14 @@
 // element on the left in the reduction order (the accumulator for a left-to-
 // right traversal).
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 #include <type_traits>
+#ifndef __HIPCC_RTC__  // nothing of the C interface is needed in device code
 #include "../../include/accelerate_hip.h"
+#endif
 
 namespace acc {
 

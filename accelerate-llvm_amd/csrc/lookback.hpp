@@ -21,7 +21,9 @@
 // running or finished workgroups: no forward-progress assumption about
 // workgroups that have not been dispatched.
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 #include "wave64.hpp"
 

@@ -3,7 +3,9 @@
 // segment start, so a scan with SegOp never carries a value across a start.
 // The identity is (identity of op, 0).
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 namespace acc {

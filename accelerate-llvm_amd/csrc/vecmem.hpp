@@ -1,6 +1,8 @@
 // 16-byte (dwordx4) global loads and stores of VEC = 16 / sizeof(T) elements.
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 namespace acc {

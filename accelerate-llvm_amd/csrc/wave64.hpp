@@ -7,7 +7,9 @@
 // Accelerate's fold/scan (accelerate-llvm-ptx/.../PTX/CodeGen/Fold.hs:425-431
 // reduces in order for the same reason).
 #pragma once
+#ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 #include "elt.hpp"
 

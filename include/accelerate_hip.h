@@ -355,6 +355,111 @@ int acc_hip_transform(acc_stream_t s, acc_type t, acc_ixmap p, acc_unop f, const
                       const void* in, int64_t in_outer, int64_t in_inner, acc_type index_type,
                       const void* index, int64_t n, void* out);
 
+/* ================================================== run-time compiled skeletons
+ * Skeletons whose scalar functions come from the program itself, compiled
+ * at run time: the reference's compile / link / execute of one skeleton node
+ * (compileForTarget PTX/Compile.hs:84-165 with the functions generated by
+ * llvmOfFun1/2, LLVM/CodeGen/Exp.hs:65-80; link PTX/Link.hs:57-121; the
+ * Execute methods PTX/Execute.hs:222-776).  A node is an S-expression:
+ *
+ *   (fold     (elt T..) (fun (a b) E) [(seed E)] [(input (fun (i) E))] [(free T..)])
+ *   (scan     (elt T..) (fun (a b) E) [(dir l2r|r2l)] [(seed E)] [prime] [(input ..)] [(free ..)])
+ *   (foldseg  (elt T..) (fun (a b) E) [(offsets i32|i64)] [(seed E)] [(input ..)] [(free ..)])
+ *   (permute  (elt T..) (fun (new old) E) (project (fun (i) E)) [(input ..)] [(free ..)])
+ *   (stencil  (in T) (out T) (radius RY RX) (boundary B) (fun () E) [(free ..)])
+ *   (generate (elt T..) (fun (i) E) | (fun (r c) E) [(free ..)])
+ *
+ * T is i8 i16 i32 i64 u8 u16 u32 u64 f32 f64 bool (Int = i64, Word = u64,
+ * Bool stored as one byte); an element has 1..4 fields, each its own array
+ * (struct of arrays).  E is an expression:
+ *   x | (const T lit) | (let ((x E) ..) E) | (cond E E E) | (tuple E ..) | (prj k E)
+ *   | (read k E)   element E (an i64) of free array k, clamped into the array
+ *   | (nb dy dx)   stencil functions: the neighbour at offset (dy, dx)
+ *   | (op E ..)    op one of the PrimFun primitives (LLVM/CodeGen/Exp.hs:244-310):
+ *       add sub mul neg abs signum quot rem quotrem div mod divmod band bor bxor
+ *       bnot shiftl shiftr rotatel rotater popcount clz ctz fdiv recip sin cos tan
+ *       asin acos atan sinh cosh tanh asinh acosh atanh exp sqrt log fpow logbase
+ *       atan2 isnan isinfinite lt gt lte gte eq neq min max land lor lnot,
+ *       (truncate T E) (round T E) (floor T E) (ceiling T E)
+ *       (fromintegral T E) (tofloating T E) (bitcast T E)
+ *   with the reference's semantics (LLVM/CodeGen/Arithmetic.hs: wrapping
+ *   integers, fminf/fmaxf, ordered float comparisons, round half away from
+ *   zero, no contraction of floating operations).
+ * B is clamp | mirror | wrap | (constant E) | (function (fun (y x) E)).
+ *
+ * Semantics per skeleton, with acc_kernel_args (outer x inner input, row-major):
+ *   fold:    out[r] = seed (+) in[r][0] (+) .. (+) in[r][inner-1]  (in element order:
+ *            the combine need only be associative); fold1 of an empty row is
+ *            ACC_ERROR_EMPTY_ARRAY, fold of one writes the seed.
+ *   scan:    l2r without seed scanl1 (inner outputs per row); with seed scanl
+ *            (inner + 1: exclusive values, the total last); r2l scanr1 / scanr
+ *            (inclusive values, the seed last); prime: scanl'/scanr' (inner
+ *            exclusive values, the totals in sums[r]).
+ *   foldseg: out[r*nseg + s] = seed (+) fold of in[r][offsets[s] .. offsets[s+1]);
+ *            empty segments get the seed (fold1Seg leaves them unwritten).
+ *   permute: out = defaults (copied unless defaults[0] == out[0] or NULL), then
+ *            out[j] = f(src[i], out[j]) for every source index i < outer*inner
+ *            whose projection j = project(i) lies in [0, nseg); other
+ *            projections are Nothing.  f must be associative and commutative.
+ *            A one-field element of 4 or 8 bytes is updated by compare-and-swap;
+ *            others by a lock per output slot (PTX/CodeGen/Permute.hs:219-258).
+ *   stencil: out[y][x] = f(neighbourhood) on a height (outer) x width (inner) array.
+ *   generate: out[i] = f(i) (or f(i / inner, i mod inner)) for i < outer*inner.
+ * A manifest input is read from in[field]; (input (fun (i) E)) computes it
+ * instead (a delayed producer fused into the consumer, LLVM/CodeGen/Sugar.hs:70-76).
+ *
+ * Code objects are cached on disk by a hash of the generated source, the
+ * compiler options and the library version (LLVM/Compile/Cache.hs:57-91):
+ * $ACC_HIP_CACHE_DIR, else $XDG_CACHE_HOME/accelerate-hip, else
+ * ~/.cache/accelerate-hip.  acc_hip_kernel_image / _from_image carry a
+ * compiled node inside a program, which then loads it without compiling
+ * (runQ embedding, PTX/Embed.hs:50-79). */
+typedef struct acc_kernel_s* acc_kernel_t;
+
+typedef enum acc_kernel_origin {
+  ACC_KERNEL_COMPILED = 0,         /* compiled by this call (and stored in the cache) */
+  ACC_KERNEL_FROM_DISK_CACHE = 1,  /* code object read from the on-disk cache */
+  ACC_KERNEL_FROM_IMAGE = 2,       /* from an image given by the caller */
+  ACC_KERNEL_LOADED = 3            /* the module was already loaded in this process */
+} acc_kernel_origin;
+
+typedef struct acc_kernel_info {
+  int origin;              /* acc_kernel_origin */
+  int lock_free_permute;   /* permute: 1 compare-and-swap, 0 lock per slot; other skeletons -1 */
+  double compile_ms;       /* time spent compiling (0 unless origin == ACC_KERNEL_COMPILED) */
+  char key[40];            /* 32 hex digits identifying the code object */
+} acc_kernel_info;
+
+typedef struct acc_kernel_args {
+  const void* in[4];        /* manifest input fields */
+  void* out[4];             /* output fields */
+  const void* defaults[4];  /* permute: default fields */
+  void* sums[4];            /* scan': totals */
+  const void* offsets;      /* foldseg: nseg + 1 offsets per row */
+  int64_t outer, inner;     /* input extents */
+  int64_t nseg;             /* foldseg: segments; permute: output size */
+  int32_t nfree;            /* free arrays given below */
+  int32_t reserved;
+  const void* free_ptr[8];  /* free array k (device) */
+  int64_t free_len[8];      /* and its element count */
+} acc_kernel_args;
+
+/* The HIP source a node translates to (*len = its length; copied when buf is given). */
+int acc_hip_node_source(const char* node, char* buf, size_t cap, size_t* len);
+/* Compile a node into the on-disk cache without a device (*origin: COMPILED or FROM_DISK_CACHE). */
+int acc_hip_node_precompile(const char* node, int* origin);
+/* Compile (or fetch from the cache) and load a node on a device. */
+int acc_hip_kernel_compile(int device, const char* node, acc_kernel_t* k);
+/* Load a node from an image made by acc_hip_kernel_image (no compilation; the
+ * image must have been made for the same node and library version). */
+int acc_hip_kernel_from_image(int device, const char* node, const void* image, size_t bytes, acc_kernel_t* k);
+/* The node's image (*bytes = size; copied when buf is given and cap suffices). */
+int acc_hip_kernel_image(acc_kernel_t k, void* buf, size_t cap, size_t* bytes);
+int acc_hip_kernel_info(acc_kernel_t k, acc_kernel_info* info);
+/* Execute the node on a stream (asynchronous, like every skeleton call). */
+int acc_hip_kernel_run(acc_kernel_t k, acc_stream_t s, const acc_kernel_args* args);
+int acc_hip_kernel_destroy(acc_kernel_t k);
+
 /* ================================================================ batched
  * Many independent scans and folds of uneven sizes in ONE persistent launch
  * (no counterpart in the reference, whose per-call path is

@@ -424,35 +424,64 @@ __device__ __forceinline__ void cas_update(W* q, const E& v, const Fn& fn) {
 // out[project(i)] = combine(src(i), out[project(i)]) for every source index i
 // whose projection lies inside the output (anything else is Nothing, the
 // element is dropped: PTX/CodeGen/Permute.hs:151-157).  The combine must be
-// associative and commutative (:67-68).  A one-field element of 4 or 8 bytes
-// is updated with a compare-and-swap loop on its bit pattern; any other
-// element takes a per-slot spin lock (the reference's mkPermute_mutex,
-// PTX/CodeGen/Permute.hs:219-258): acquire at agent scope, read, combine and
-// write the fields, release.  A lane retries only its own lock, and the lane
-// that takes a lock releases it in the same loop iteration, so the lanes of a
-// wave that contend for one slot go through it one at a time.
+// associative and commutative (:67-68), so the lanes of a wave that update
+// the same slot first combine their values (in lane order, a masked wave
+// reduction) and one lane applies the result: a slot is updated at most once
+// per wave and grid step, however skewed the indices.  A one-field element of
+// 4 or 8 bytes is updated with a compare-and-swap loop on its bit pattern;
+// any other element under a per-slot spin lock (the reference's
+// mkPermute_mutex, PTX/CodeGen/Permute.hs:219-258): the lock word is polled
+// with plain loads and taken with one swap at agent scope (acquire), the
+// fields are read, combined and written, and the lock is released.  Only one
+// lane of a wave waits at a time, so no lane waits on a lane of its own wave.
+// A wait is bounded (kLockTries polls with back-off, seconds): a lane that
+// cannot take a lock in that time drops its update rather than hang.
+constexpr uint32_t kLockTries = 1u << 22;
+
 template <typename E, typename Fn, typename Ld, typename Proj, typename Out, bool CAS>
 __device__ void permute(const Args& a, const Fn& fn, const Ld& ld, const Proj& proj, const Out& out) {
+  const int lane = lane_id();
   const int64_t nsrc = a.outer * a.inner;
   const int64_t stride = (int64_t)gridDim.x * BLOCK;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nsrc; i += stride) {
-    const int64_t j = proj(i);
-    if (j < 0 || j >= a.nseg) continue;
-    const E v = ld(i);
-    if constexpr (CAS) {
-      out.cas_update(j, v, fn);
-    } else {
-      uint32_t* lk = a.locks + j;
-      bool done = false;
-      while (!done) {
-        if (__hip_atomic_exchange(lk, 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-          out.store(j, fn(v, out.load(j)));
-          __hip_atomic_store(lk, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-          done = true;
+  // wave-uniform trip count: every lane of the wave runs every step (the wave
+  // reductions below need the whole wave)
+  for (int64_t base = (int64_t)blockIdx.x * BLOCK + (threadIdx.x & ~63); base < nsrc; base += stride) {
+    const int64_t i = base + lane;
+    int64_t j = -1;
+    E v{};
+    if (i < nsrc) {
+      j = proj(i);
+      if (j >= 0 && j < a.nseg) v = ld(i);
+      else j = -1;
+    }
+    bool pending = j >= 0;
+    while (true) {
+      const uint64_t act = __ballot(pending);
+      if (act == 0) break;
+      const int leader = __builtin_ctzll(act);
+      const int64_t jl = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)j >> 32), leader) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)j, leader));
+      const bool mine = pending && j == jl;
+      const Maybe<E> r = acc::wave_reduce_m(mine ? just(v) : none<E>(), fn);
+      if (lane == leader) {
+        if constexpr (CAS) {
+          out.cas_update(jl, r.v, fn);
         } else {
-          __builtin_amdgcn_s_sleep(1);
+          uint32_t* lk = a.locks + jl;
+          for (uint32_t tries = 0; tries < kLockTries; ++tries) {
+            if (acc::ld_relaxed(lk) == 0u &&
+                __hip_atomic_exchange(lk, 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+              out.store(jl, fn(r.v, out.load(jl)));
+              __hip_atomic_store(lk, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+            if (tries < 4) __builtin_amdgcn_s_sleep(1);
+            else if (tries < 16) __builtin_amdgcn_s_sleep(4);
+            else __builtin_amdgcn_s_sleep(16);
+          }
         }
       }
+      pending = pending && !mine;
     }
   }
 }

@@ -349,8 +349,12 @@ def _combiner(interp, fun, nf):
     return f
 
 
-def _cast(fields, types):
-    return [np.asarray(v).astype(storage(t)) for v, t in zip(fields, types)]
+def _cast(fields, types, n=None):
+    """Fields in their storage types; with n, constant (0-d) fields broadcast to n elements."""
+    out = [np.asarray(v).astype(storage(t)) for v, t in zip(fields, types)]
+    if n is not None:
+        out = [np.broadcast_to(v, (n,)).copy() if v.ndim == 0 else v for v in out]
+    return out
 
 
 def _seed(interp, items, types):
@@ -364,7 +368,7 @@ def _input(interp, items, inputs, n, types):
     if "input" in items:
         ps, body = fun_params(items["input"][1])
         i = np.arange(n, dtype=np.int64)
-        return _cast(interp.ev(body, {ps[0]: [i]}), types)
+        return _cast(interp.ev(body, {ps[0]: [i]}), types, n)
     return [np.asarray(x).astype(storage(t)) for x, t in zip(inputs, types)]
 
 
@@ -567,7 +571,7 @@ def generate(node_text, outer, inner, free=()):
     ps, body = fun_params(fun)
     i = np.arange(outer * inner, dtype=np.int64)
     env = {ps[0]: [i]} if len(ps) == 1 else {ps[0]: [i // max(inner, 1)], ps[1]: [i % max(inner, 1)]}
-    return _cast(it.ev(body, env), types)
+    return _cast(it.ev(body, env), types, outer * inner)
 
 
 def evaluate(expr_text, env_arrays, free=()):

@@ -160,6 +160,12 @@ class KernelArgs(ctypes.Structure):
 
 KERNEL_COMPILED, KERNEL_FROM_DISK_CACHE, KERNEL_FROM_IMAGE, KERNEL_LOADED = range(4)
 
+
+class TableStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "uploads", "downloads", "evictions", "nursery_hits", "oom_retries", "bytes_up", "bytes_down",
+        "bytes_evicted", "resident_bytes", "nursery_bytes", "entries")]
+
 # exported symbol -> (restype, argtypes); every declaration in the header
 SIGNATURES = {
     "acc_hip_last_error": (c_char_p, []),
@@ -230,6 +236,15 @@ SIGNATURES = {
     "acc_hip_kernel_info": (c_int, [c_void_p, POINTER(KernelInfo)]),
     "acc_hip_kernel_run": (c_int, [c_void_p, c_void_p, POINTER(KernelArgs)]),
     "acc_hip_kernel_destroy": (c_int, [c_void_p]),
+    "acc_hip_table_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "acc_hip_table_use": (c_int, [c_void_p, ctypes.c_uint64, c_void_p, c_size_t, c_void_p, POINTER(c_void_p)]),
+    "acc_hip_table_alloc": (c_int, [c_void_p, ctypes.c_uint64, c_void_p, c_size_t, POINTER(c_void_p)]),
+    "acc_hip_table_get": (c_int, [c_void_p, ctypes.c_uint64, c_void_p, POINTER(c_void_p)]),
+    "acc_hip_table_mark": (c_int, [c_void_p, ctypes.c_uint64, c_void_p, c_int]),
+    "acc_hip_table_peek": (c_int, [c_void_p, ctypes.c_uint64, POINTER(c_int)]),
+    "acc_hip_table_free": (c_int, [c_void_p, ctypes.c_uint64]),
+    "acc_hip_table_stats": (c_int, [c_void_p, POINTER(TableStats)]),
+    "acc_hip_table_destroy": (c_int, [c_void_p]),
     "acc_hip_batch_create": (c_int, [c_int, POINTER(BatchProblem), c_int64, c_int, POINTER(c_void_p)]),
     "acc_hip_batch_launch": (c_int, [c_void_p, c_void_p]),
     "acc_hip_batch_tile": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_uint32)]),
@@ -739,6 +754,74 @@ class Kernel:
     def destroy(self):
         if self.handle:
             lib().acc_hip_kernel_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ memory table
+class DevRef:
+    """A raw device pointer with an element type and count (not owning)."""
+
+    def __init__(self, ptr, count, dtype, device=0):
+        self.ptr, self.count, self.dtype, self.device = ptr, count, np.dtype(dtype), device
+        self.nbytes = self.count * self.dtype.itemsize
+
+
+class Table:
+    """The remote memory table (acc_hip_table_*): numpy arrays by key."""
+
+    def __init__(self, device=0):
+        h = c_void_p()
+        check(lib().acc_hip_table_create(device, ctypes.byref(h)))
+        self.handle, self.device = h, device
+        self.host = {}  # key -> numpy array kept alive while registered
+
+    def use(self, stream, key, a: np.ndarray) -> DevRef:
+        a = np.ascontiguousarray(a)
+        self.host[key] = a
+        p = c_void_p()
+        check(lib().acc_hip_table_use(self.handle, key, a.ctypes.data_as(c_void_p), a.nbytes, stream.handle,
+                                      ctypes.byref(p)))
+        return DevRef(p.value, a.size, a.dtype, self.device)
+
+    def alloc(self, key, count, dtype, host_backing=True) -> DevRef:
+        h = np.empty(count, dtype) if host_backing else None
+        self.host[key] = h
+        p = c_void_p()
+        check(lib().acc_hip_table_alloc(self.handle, key, h.ctypes.data_as(c_void_p) if h is not None else None,
+                                        count * np.dtype(dtype).itemsize, ctypes.byref(p)))
+        return DevRef(p.value, count, dtype, self.device)
+
+    def get(self, stream, key, count, dtype) -> DevRef:
+        p = c_void_p()
+        check(lib().acc_hip_table_get(self.handle, key, stream.handle, ctypes.byref(p)))
+        return DevRef(p.value, count, dtype, self.device)
+
+    def mark(self, stream, key, writes=False):
+        check(lib().acc_hip_table_mark(self.handle, key, stream.handle, 1 if writes else 0))
+
+    def peek(self, key) -> tuple:
+        c = c_int()
+        check(lib().acc_hip_table_peek(self.handle, key, ctypes.byref(c)))
+        return self.host[key], bool(c.value)
+
+    def free(self, key):
+        check(lib().acc_hip_table_free(self.handle, key))
+        self.host.pop(key, None)
+
+    def stats(self) -> TableStats:
+        st = TableStats()
+        check(lib().acc_hip_table_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def destroy(self):
+        if self.handle:
+            lib().acc_hip_table_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -460,6 +460,44 @@ int acc_hip_kernel_info(acc_kernel_t k, acc_kernel_info* info);
 int acc_hip_kernel_run(acc_kernel_t k, acc_stream_t s, const acc_kernel_args* args);
 int acc_hip_kernel_destroy(acc_kernel_t k);
 
+/* ===================================================== remote memory table
+ * Host arrays, named by a key the host chooses (its array's identity), mapped
+ * to device buffers: the reference's per-context memory table
+ * (PTX/Array/Table.hs:35-49 over the core's LRU table; instance Remote,
+ * PTX/Array/Data.hs:58-152).
+ *   use   : upload a host array (if not resident) and freeze it: the host copy
+ *           stays authoritative, so the device copy may be evicted and
+ *           uploaded again later (useRemote, PTX/Array/Prim.hs:89-100).  The
+ *           host buffer must stay valid while the key is in the table.
+ *   alloc : a device buffer for a result (allocateRemote); `host`, if not
+ *           NULL, is where it is copied back (lazily, or before an eviction).
+ *   get   : the device pointer, re-uploading an evicted entry.
+ *   mark  : work queued on `s` uses the entry (writes != 0: and writes it);
+ *           it is not evicted, copied back or recycled before that work ends.
+ *   peek  : copy a result back to its host buffer if the device copy is newer
+ *           (copyToHostLazy, PTX/Array/Data.hs:91-152); *copied says whether it was.
+ *   free  : the host array is gone (a finaliser); its block waits for reuse.
+ * Any allocation that does not fit evicts least recently used entries (copying
+ * dirty ones back first) and retries (PTX/Array/Remote.hs:63-74); entries that
+ * exist only on the device are never evicted, and when nothing is left to evict
+ * the call returns ACC_ERROR_OUT_OF_MEMORY.  Blocks are 4096-byte multiples
+ * (PTX/Array/Remote.hs:101). */
+typedef struct acc_table_s* acc_table_t;
+typedef struct acc_table_stats {
+  uint64_t uploads, downloads, evictions, nursery_hits, oom_retries;
+  uint64_t bytes_up, bytes_down, bytes_evicted;
+  uint64_t resident_bytes, nursery_bytes, entries;
+} acc_table_stats;
+int acc_hip_table_create(int device, acc_table_t* t);
+int acc_hip_table_use(acc_table_t t, uint64_t key, const void* host, size_t bytes, acc_stream_t s, void** dev);
+int acc_hip_table_alloc(acc_table_t t, uint64_t key, void* host, size_t bytes, void** dev);
+int acc_hip_table_get(acc_table_t t, uint64_t key, acc_stream_t s, void** dev);
+int acc_hip_table_mark(acc_table_t t, uint64_t key, acc_stream_t s, int writes);
+int acc_hip_table_peek(acc_table_t t, uint64_t key, int* copied);
+int acc_hip_table_free(acc_table_t t, uint64_t key);
+int acc_hip_table_stats(acc_table_t t, acc_table_stats* st);
+int acc_hip_table_destroy(acc_table_t t);
+
 /* ================================================================ batched
  * Many independent scans and folds of uneven sizes in ONE persistent launch
  * (no counterpart in the reference, whose per-call path is

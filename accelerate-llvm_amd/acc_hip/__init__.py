@@ -87,10 +87,15 @@ class BatchProblem(ctypes.Structure):
         ("in_", c_void_p),
         ("out", c_void_p),
         ("seed", ctypes.c_uint64),
+        ("aux", c_void_p),
+        ("m", ctypes.c_int64),
+        ("aux_type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("fill", ctypes.c_uint64),
     ]
 
 
-BATCH_SCAN, BATCH_FOLD = 0, 1
+BATCH_SCAN, BATCH_FOLD, BATCH_FOLD_SEG, BATCH_HISTOGRAM = 0, 1, 2, 3
 
 REQ_FOLD, REQ_SCAN, REQ_FOLD_SEG, REQ_PERMUTE, REQ_STENCIL2D = range(5)
 
@@ -247,7 +252,10 @@ SIGNATURES = {
     "acc_hip_table_destroy": (c_int, [c_void_p]),
     "acc_hip_batch_create": (c_int, [c_int, POINTER(BatchProblem), c_int64, c_int, POINTER(c_void_p)]),
     "acc_hip_batch_launch": (c_int, [c_void_p, c_void_p]),
+    "acc_hip_batch_launch_plain": (c_int, [c_void_p, c_void_p]),
     "acc_hip_batch_tile": (c_int, [c_void_p, POINTER(c_int), POINTER(ctypes.c_uint32)]),
+    "acc_hip_batch_shape": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                                    POINTER(ctypes.c_uint32)]),
     "acc_hip_batch_destroy": (c_int, [c_void_p]),
     "acc_hip_server_create": (c_int, [c_int, c_int, c_size_t, POINTER(c_void_p)]),
     "acc_hip_server_submit": (c_int, [c_void_p, POINTER(Request), c_int64, c_int, POINTER(ctypes.c_uint64)]),
@@ -535,17 +543,29 @@ class Batch:
     """Many scans/folds in one persistent launch (acc_hip_batch_*)."""
 
     def __init__(self, device, problems, tile=0):
-        """problems: list of (kind, op, seed_or_None, src DeviceArray/View, n, out DeviceArray/View)."""
+        """problems: list of (kind, op, seed_or_None, src, n, out[, aux, m, fill]): src/out
+        DeviceArray/View; foldSeg: aux = offsets, m = segments; histogram: src = indices,
+        out = m bins of the result type, seed = their initial value, fill = the source value."""
         arr = (BatchProblem * len(problems))()
-        for i, (kind, op, seed, src, n, out) in enumerate(problems):
+        for i, prob in enumerate(problems):
+            kind, op, seed, src, n, out = prob[:6]
+            aux, m, fill = (list(prob[6:]) + [None, 0, 0])[:3]
+            dt = out.dtype if kind == BATCH_HISTOGRAM else src.dtype
             arr[i].kind = kind
-            arr[i].type = acc_type(src.dtype)
+            arr[i].type = acc_type(dt)
             arr[i].op = op
             arr[i].has_seed = 0 if seed is None else 1
             arr[i].n = n
-            arr[i].in_ = src.ptr
+            arr[i].in_ = src.ptr if src is not None else None
             arr[i].out = out.ptr
-            arr[i].seed = 0 if seed is None else _bits(src.dtype, seed)
+            arr[i].seed = 0 if seed is None else _bits(dt, seed)
+            if aux is not None:
+                arr[i].aux = aux.ptr
+                arr[i].aux_type = acc_type(aux.dtype)
+            elif kind == BATCH_HISTOGRAM:
+                arr[i].aux_type = acc_type(src.dtype)
+            arr[i].m = m
+            arr[i].fill = _bits(dt, fill) if kind == BATCH_HISTOGRAM else 0
         h = c_void_p()
         check(lib().acc_hip_batch_create(device, arr, len(problems), tile, ctypes.byref(h)))
         self.handle = h
@@ -553,10 +573,21 @@ class Batch:
     def launch(self, stream):
         check(lib().acc_hip_batch_launch(self.handle, stream.handle))
 
+    def launch_plain(self, stream):
+        """The same problems, one plain C call each (the per-call path)."""
+        check(lib().acc_hip_batch_launch_plain(self.handle, stream.handle))
+
     def tile(self):
         k, t = c_int(), ctypes.c_uint32()
         check(lib().acc_hip_batch_tile(self.handle, ctypes.byref(k), ctypes.byref(t)))
         return k.value, t.value
+
+    def shape(self):
+        """(workgroup size, vectors per thread, tiles per ticket, tickets)"""
+        b, k, r, t = c_int(), c_int(), c_int(), ctypes.c_uint32()
+        check(lib().acc_hip_batch_shape(self.handle, ctypes.byref(b), ctypes.byref(k), ctypes.byref(r),
+                                        ctypes.byref(t)))
+        return b.value, k.value, r.value, t.value
 
     def destroy(self):
         if self.handle:

@@ -1,20 +1,38 @@
-// Batched execution: many independent scans and folds of uneven sizes in ONE
-// persistent launch, capturable as a hipGraph.
+// Batched execution: many independent scans, folds, segmented folds and
+// histograms of uneven sizes in ONE persistent launch, capturable as a
+// hipGraph.
 //
-// The plain path costs one kernel launch (and the reference's path 1-4
-// launches plus host marshalling, PTX/Execute.hs:835-881) per problem, so
-// small problems are dominated by launch latency.  Here every problem is cut
-// into tiles; the tiles of all problems form one ticket space; persistent
-// workgroups take tiles in ticket order and dispatch on the problem's class
-// (kind x element type x combine function).  Each problem owns a slice of the
-// status words, so its scan look-back never crosses into another problem.
-// A fold's intermediate tiles only publish their aggregate; its last tile
-// looks back over all of them and writes the result with the seed on the
-// left.
+// The plain path costs one or more kernel launches per problem (and the
+// reference's path 1-4 launches plus host marshalling, PTX/Execute.hs:835-881),
+// so small problems are dominated by launch latency.  Here every problem is
+// cut into runs of work; the runs of all problems form one ticket space;
+// persistent workgroups take runs in ticket order (a run→problem map gives
+// the problem in one load) and dispatch on the problem's class (kind x
+// element type x combine function):
+//   fold         a run is R consecutive tiles of BLOCK*K*VEC elements,
+//                reduced in registers as one piece; intermediate runs publish
+//                their aggregates and the last run looks back and writes the
+//                result with the seed on the left
+//   scan         a problem of at most R tiles is one run; a larger one has
+//                one tile per run, each a tile of the single-pass look-back
+//                publishing its inclusive prefix
+//                Each problem owns a slice of the status words, so no
+//                look-back crosses into another.
+//   foldSeg      a run is 64 segments; each wave folds 16 of them in order
+//                (the reference's warp per segment, PTX/CodeGen/FoldSeg.hs:274-458)
+//   histogram    a run is 64 Ki indices folded into a private LDS copy of the
+//                bins, then flushed with one global atomic per non-zero bin;
+//                the problem's first run writes the initial bins and
+//                publishes a tagged flag, which every run waits for before
+//                it flushes (the first run is ticketed before the others, so
+//                the wait ends).
+// One ticket per run instead of per tile: the ticket counter, which costs
+// ~12 ns per ticket at the device (measured), bounded the first version of
+// this kernel (309 k tiles = 3.8 ms); runs divide the tickets by R.
 //
-// The tile size (dwordx4 vectors per thread) is chosen by an autotuner that
-// times the candidates on the batch itself and remembers the winner per
-// batch signature in a small text file.
+// The workgroup size, the tile size (vectors per thread) and the run length
+// are chosen by an autotuner that times the candidates on the batch itself
+// and remembers the winner per batch signature in a small text file.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -23,30 +41,41 @@
 #include <algorithm>
 #include <chrono>
 #include "scan_tile.hpp"
+#include "batch_internal.hpp"
 
 namespace acc {
 
-enum : int32_t { BK_SCAN = 0, BK_FOLD = 1 };
+enum : int32_t { BK_SCAN = 0, BK_FOLD = 1, BK_FOLD_SEG = 2, BK_HIST = 3 };
 
-struct BatchDesc {
-  int32_t cls;        // kind * 8 + type index * 2 + op index
+struct BatchDesc {  // one device descriptor per problem (96 bytes, asserted below)
+  int32_t cls;          // kind * 8 + type index * 2 + op index
   uint32_t has_seed;
   int64_t n;
   const void* in;
   void* out;
-  uint64_t seed;      // bit pattern
+  uint64_t seed;        // bit pattern
   uint64_t status_off;  // first status word of this problem
-  uint32_t ntiles;
-  uint32_t tile0;     // first global ticket of this problem
+  uint32_t ntiles;      // scan/fold: tiles; foldSeg: segments; histogram: indices / HIST_RUN
+  uint32_t run0;        // first ticket of this problem
+  uint32_t R;           // scan/fold: tiles per run
+  uint32_t aux64;       // foldSeg offsets / histogram indices are Int64
+  const void* aux;      // foldSeg: offsets
+  int64_t m;            // foldSeg: segments; histogram: bins
+  uint64_t fill;        // histogram: constant source value
+  uint64_t reserved;    // pads to the host's descriptor stride (96 bytes)
 };
 
 struct BatchArgs {
   const BatchDesc* desc;
-  const uint32_t* tile_prob;  // global tile -> problem
-  uint32_t total_tiles;
+  const uint32_t* run_prob;  // ticket -> problem
+  uint32_t total_runs;
   SyncCtl* ctl;
   uint64_t* status;
 };
+
+constexpr int SEG_RUN = 64;          // segments per foldSeg run
+constexpr int64_t HIST_RUN = 65536;  // indices per histogram run
+constexpr int HIST_MAX_BINS = 4096;  // bins held in LDS (8-byte bins: 32 KiB)
 
 template <typename T> __device__ __forceinline__ T bits_to(uint64_t b) {
   T v;
@@ -54,10 +83,14 @@ template <typename T> __device__ __forceinline__ T bits_to(uint64_t b) {
   return v;
 }
 
-// Fold of one tile; the problem's last tile finishes the fold.
-template <typename T, typename Op, int BLOCK, int K, bool MASKED>
-__device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, uint32_t t, uint32_t epoch,
-                                          T* s_wave) {
+// Fold of run r: tiles [r*R, min(ntiles, r*R + R)) reduced in registers as
+// one piece, then one block reduction and one status slot per run; the
+// problem's last run looks back over the runs before it and writes the
+// result with the seed on the left.  Every run depends only on runs with
+// lower tickets, so the look-back ends, and no run waits inside its tiles.
+template <typename T, typename Op, int BLOCK, int K>
+__device__ __forceinline__ void fold_run(const BatchDesc& d, uint64_t* status, uint32_t r, uint32_t epoch,
+                                         T* s_wave) {
   constexpr int VEC = Vec<T>::N;
   constexpr int W = BLOCK / 64;
   constexpr int64_t TILE = (int64_t)BLOCK * K * VEC;
@@ -65,36 +98,36 @@ __device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, 
   Op op{};
   const int lane = lane_id(), w = threadIdx.x / 64;
   const T* in = (const T*)d.in;
-  const int64_t base = (int64_t)t * TILE;
   const T id = identity<T>(op);
+  const uint32_t nruns = (d.ntiles + d.R - 1) / d.R;
+  const uint32_t t0 = r * d.R, t1 = min(d.ntiles, t0 + d.R);
   T acc = id;
-  if (!MASKED) {
+  for (uint32_t t = t0; t < t1; ++t) {
+    const int64_t base = (int64_t)t * TILE;
     Vec<T> v[K];
+    if (base + TILE <= d.n) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = load16_stream(in + base + ((int64_t)(w * K + k) * 64 + lane) * VEC);
-    T r = v[0].e[0];
+      for (int k = 0; k < K; ++k) v[k] = load16_stream(in + base + ((int64_t)(w * K + k) * 64 + lane) * VEC);
 #pragma unroll
-    for (int j = 1; j < VEC; ++j) r = op(r, v[0].e[j]);
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-    for (int k = 1; k < K; ++k)
+        for (int j = 0; j < VEC; ++j) acc = op(acc, v[k].e[j]);
+    } else {
+      // the problem's last tile: a vector past the end re-reads the vector
+      // holding the last element (an aligned 16-byte block with a valid
+      // element never crosses a page), so every load is one unconditional
+      // dwordx4, and the elements past the end are replaced by the identity
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) r = op(r, v[k].e[j]);
-    acc = r;
-  } else {
-    // the problem's last tile: a vector past the end re-reads the vector
-    // holding the last element (an aligned 16-byte block with a valid element
-    // never crosses a page), so every load is one unconditional dwordx4
-    Vec<T> v[K];
+      for (int k = 0; k < K; ++k) {
+        const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
+        v[k] = load16_stream(in + min(e0, (d.n - 1) / VEC * VEC));
+      }
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
-      v[k] = load16_stream(in + min(e0, (d.n - 1) / VEC * VEC));
-    }
+      for (int k = 0; k < K; ++k) {
+        const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t e0 = base + ((int64_t)(w * K + k) * 64 + lane) * VEC;
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) acc = op(acc, e0 + j < d.n ? v[k].e[j] : id);
+        for (int j = 0; j < VEC; ++j) acc = op(acc, e0 + j < d.n ? v[k].e[j] : id);
+      }
     }
   }
   const T wr = wave_reduce(acc, op);
@@ -104,41 +137,186 @@ __device__ __forceinline__ void fold_tile(const BatchDesc& d, uint64_t* status, 
     T agg = id;
 #pragma unroll
     for (int i = 0; i < W; ++i) agg = op(agg, s_wave[i]);
-    uint64_t* slot = status + (size_t)t * NW;
-    const bool first = t == 0, last = t + 1 == d.ntiles;
+    uint64_t* slot = status + (size_t)r * NW;
+    const bool first = r == 0, last = r + 1 == nruns;
     if (!last) {
       if (lane == 0) Tagged<T>::publish(slot, agg, make_tag(epoch, first ? ST_PREFIX : ST_AGGREGATE));
     } else {
-      T r = agg;
-      if (!first) r = op(wave_lookback<T>(status, t, epoch, op), r);
-      if (d.has_seed) r = op(bits_to<T>(d.seed), r);  // seed on the left
-      if (lane == 0) *(T*)d.out = r;
+      T res = agg;
+      if (!first) res = op(wave_lookback<T>(status, r, epoch, op), res);
+      if (d.has_seed) res = op(bits_to<T>(d.seed), res);  // seed on the left
+      if (lane == 0) *(T*)d.out = res;
     }
   }
 }
 
+// Scan of run r.  A problem of at most R tiles is one run, scanned tile
+// after tile by one workgroup (each tile's look-back finds the inclusive
+// prefix the same workgroup published for the tile before); a larger problem
+// has one tile per run.  (Runs of several consecutive tiles of a large scan
+// chain every run behind the previous one -- a run's first tile needs the
+// aggregate of the previous run's last tile -- which makes the problem's
+// scan serial, one tile at a time, with every later run's workgroup holding
+// its slot while it waits.)
 template <typename T, typename Op, int BLOCK, int K>
-__device__ __forceinline__ void batch_tile(const BatchDesc& d, int kind, uint64_t* status, uint32_t t,
-                                           uint32_t epoch, unsigned char* s_raw) {
+__device__ __forceinline__ void scan_run(const BatchDesc& d, uint64_t* status, uint32_t r, uint32_t epoch,
+                                         unsigned char* s_raw) {
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
   T* s_wave = (T*)s_raw;
   T* s_prefix = s_wave + BLOCK / 64;
-  const int64_t base = (int64_t)t * TILE;
-  const bool full = base + TILE <= d.n;
-  if (kind == BK_SCAN) {
+  const uint32_t t0 = r * d.R, t1 = min(d.ntiles, t0 + d.R);
+  for (uint32_t t = t0; t < t1; ++t) {
+    const int64_t base = (int64_t)t * TILE;
     // scanl1 without a seed; with a seed the exclusive scanl (total at out[n])
     TileIO<T> io{(const T*)d.in, (T*)d.out, d.n, 0u, d.ntiles - 1, d.has_seed, d.has_seed, bits_to<T>(d.seed),
                  d.has_seed ? (T*)d.out + d.n : nullptr, nullptr, status};
-    if (full) scan_tile<T, Op, BLOCK, K, false, true, false>(io, Op{}, t, base, epoch, s_wave, s_prefix);
+    if (base + TILE <= d.n) scan_tile<T, Op, BLOCK, K, false, true, false>(io, Op{}, t, base, epoch, s_wave, s_prefix);
     else scan_tile<T, Op, BLOCK, K, false, true, true>(io, Op{}, t, base, epoch, s_wave, s_prefix);
+    __syncthreads();
+  }
+}
+
+template <typename T, typename Op, int BLOCK, int K>
+__device__ __forceinline__ void scan_fold_run(const BatchDesc& d, int kind, uint64_t* status, uint32_t r,
+                                              uint32_t epoch, unsigned char* s_raw) {
+  if (kind == BK_SCAN) scan_run<T, Op, BLOCK, K>(d, status, r, epoch, s_raw);
+  else fold_run<T, Op, BLOCK, K>(d, status, r, epoch, (T*)s_raw);
+}
+
+// foldSeg / fold1Seg: wave w folds segments [r*SEG_RUN + w*16, +16) in order,
+// 256 elements per step (4 consecutive per lane, an in-lane fold, then the
+// in-order wave reduction); empty segments get the seed (fold1Seg: left as
+// they are).
+template <typename T, typename Op, int BLOCK>
+__device__ __forceinline__ void fold_seg_run(const BatchDesc& d, uint32_t r) {
+  constexpr int W = BLOCK / 64;
+  constexpr int PER = SEG_RUN / W;
+  Op op{};
+  const int lane = lane_id(), w = threadIdx.x / 64;
+  const T* in = (const T*)d.in;
+  const int64_t s0 = (int64_t)r * SEG_RUN + w * PER;
+  for (int k = 0; k < PER; ++k) {
+    const int64_t s = s0 + k;
+    if (s >= d.m) break;  // wave-uniform
+    const int64_t lo = d.aux64 ? ((const int64_t*)d.aux)[s] : ((const int32_t*)d.aux)[s];
+    const int64_t hi = d.aux64 ? ((const int64_t*)d.aux)[s + 1] : ((const int32_t*)d.aux)[s + 1];
+    Maybe<T> acc;
+    acc.v = T();
+    acc.ok = d.has_seed;
+    if (d.has_seed) acc.v = bits_to<T>(d.seed);
+    for (int64_t i = lo; i < hi; i += 256) {
+      const int64_t e = i + lane * 4;
+      Maybe<T> x;
+      x.v = T();
+      x.ok = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (e + j < hi) {
+          Maybe<T> y;
+          y.v = in[e + j];
+          y.ok = 1;
+          x = mcombine(op, x, y);
+        }
+      acc = mcombine(op, acc, wave_reduce_m(x, op));
+    }
+    if (lane == 0 && acc.ok) ((T*)d.out)[s] = acc.v;
+  }
+}
+
+// The histogram: out[idx[i]] = out[idx[i]] (+) fill for 0 <= idx[i] < m, with
+// out initially `seed` in every bin (permute (+) of a constant source).
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v) {
+  if constexpr (std::is_floating_point<T>::value) atomicAdd(p, v);
+  else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void global_add(T* p, T v) {
+  if constexpr (std::is_floating_point<T>::value) unsafeAtomicAdd(p, v);
+  else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, typename I, int BLOCK>
+__device__ __forceinline__ void hist_run(const BatchDesc& d, uint64_t* status, uint32_t r, uint32_t epoch,
+                                         T* bins) {
+  T* out = (T*)d.out;
+  const int64_t m = d.m;
+  const T fill = bits_to<T>(d.fill);
+  const uint32_t ready = make_tag(epoch, ST_PREFIX);
+  if (r == 0) {
+    // the first run initialises the bins, makes the stores visible to every
+    // XCD, then publishes the flag
+    const T z = bits_to<T>(d.seed);
+    for (int64_t b = threadIdx.x; b < m; b += BLOCK) out[b] = z;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(status, (uint64_t)ready << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int64_t b = threadIdx.x; b < m; b += BLOCK) bins[b] = T(0);
+  __syncthreads();
+  const I* idx = (const I*)d.in;
+  const int64_t i0 = (int64_t)r * HIST_RUN, i1 = min(d.n, i0 + HIST_RUN);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    const int64_t j = (int64_t)idx[i];
+    if (j >= 0 && j < m) lds_add(&bins[j], fill);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    while ((uint32_t)(__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) != ready)
+      __builtin_amdgcn_s_sleep(2);
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < m; b += BLOCK) {
+    const T v = bins[b];
+    if (v != T(0)) global_add(&out[b], v);
+  }
+  __syncthreads();
+}
+
+template <int BLOCK, int K>
+__device__ __forceinline__ void batch_run(const BatchDesc& d, uint64_t* st, uint32_t r, uint32_t epoch,
+                                          unsigned char* s_raw, unsigned char* s_dyn) {
+  const int kind = d.cls >> 3;
+  const int c = d.cls & 7;
+  if (kind <= BK_FOLD) {
+    switch (c) {
+      case 0: scan_fold_run<int32_t, OpAdd, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      case 1: scan_fold_run<int32_t, OpMax, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      case 2: scan_fold_run<int64_t, OpAdd, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      case 3: scan_fold_run<int64_t, OpMax, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      case 4: scan_fold_run<float, OpAdd, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      case 5: scan_fold_run<float, OpMax, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      case 6: scan_fold_run<double, OpAdd, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+      default: scan_fold_run<double, OpMax, BLOCK, K>(d, kind, st, r, epoch, s_raw); break;
+    }
+  } else if (kind == BK_FOLD_SEG) {
+    switch (c) {
+      case 0: fold_seg_run<int32_t, OpAdd, BLOCK>(d, r); break;
+      case 1: fold_seg_run<int32_t, OpMax, BLOCK>(d, r); break;
+      case 2: fold_seg_run<int64_t, OpAdd, BLOCK>(d, r); break;
+      case 3: fold_seg_run<int64_t, OpMax, BLOCK>(d, r); break;
+      case 4: fold_seg_run<float, OpAdd, BLOCK>(d, r); break;
+      case 5: fold_seg_run<float, OpMax, BLOCK>(d, r); break;
+      case 6: fold_seg_run<double, OpAdd, BLOCK>(d, r); break;
+      default: fold_seg_run<double, OpMax, BLOCK>(d, r); break;
+    }
   } else {
-    if (full) fold_tile<T, Op, BLOCK, K, false>(d, status, t, epoch, s_wave);
-    else fold_tile<T, Op, BLOCK, K, true>(d, status, t, epoch, s_wave);
+    const bool i64 = d.aux64 != 0;
+    switch (c >> 1) {
+      case 0: i64 ? hist_run<int32_t, int64_t, BLOCK>(d, st, r, epoch, (int32_t*)s_dyn)
+                  : hist_run<int32_t, int32_t, BLOCK>(d, st, r, epoch, (int32_t*)s_dyn); break;
+      case 1: i64 ? hist_run<int64_t, int64_t, BLOCK>(d, st, r, epoch, (int64_t*)s_dyn)
+                  : hist_run<int64_t, int32_t, BLOCK>(d, st, r, epoch, (int64_t*)s_dyn); break;
+      case 2: i64 ? hist_run<float, int64_t, BLOCK>(d, st, r, epoch, (float*)s_dyn)
+                  : hist_run<float, int32_t, BLOCK>(d, st, r, epoch, (float*)s_dyn); break;
+      default: i64 ? hist_run<double, int64_t, BLOCK>(d, st, r, epoch, (double*)s_dyn)
+                   : hist_run<double, int32_t, BLOCK>(d, st, r, epoch, (double*)s_dyn); break;
+    }
   }
 }
 
 template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
+  extern __shared__ __align__(16) unsigned char s_dyn[];  // histogram bins (dynamic LDS)
   __shared__ uint64_t s_tick;
   __shared__ __align__(16) unsigned char s_raw[(BLOCK / 64 + 1) * sizeof(double)];
   if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
@@ -146,21 +324,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
   const uint64_t tk0 = s_tick;
   const uint32_t epoch = (uint32_t)(tk0 >> 32);
   uint32_t g = (uint32_t)tk0;
-  while (g < a.total_tiles) {
-    const BatchDesc d = a.desc[a.tile_prob[g]];
-    const uint32_t t = g - d.tile0;
-    uint64_t* st = a.status + d.status_off;
-    const int kind = d.cls >> 3;
-    switch (d.cls & 7) {
-      case 0: batch_tile<int32_t, OpAdd, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      case 1: batch_tile<int32_t, OpMax, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      case 2: batch_tile<int64_t, OpAdd, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      case 3: batch_tile<int64_t, OpMax, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      case 4: batch_tile<float, OpAdd, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      case 5: batch_tile<float, OpMax, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      case 6: batch_tile<double, OpAdd, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-      default: batch_tile<double, OpMax, BLOCK, K>(d, kind, st, t, epoch, s_raw); break;
-    }
+  while (g < a.total_runs) {
+    const BatchDesc d = a.desc[a.run_prob[g]];
+    batch_run<BLOCK, K>(d, a.status + d.status_off, g - d.run0, epoch, s_raw, s_dyn);
     __syncthreads();
     if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
     __syncthreads();
@@ -169,26 +335,21 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
   block_retire(a.ctl, gridDim.x, epoch);
 }
 
-constexpr int BATCH_BLOCK = 256;
-static const int kBatchK[] = {1, 2, 4};
-
-}  // namespace acc
-
-// ----------------------------------------------------------------- host side
-using namespace acc;
-
-struct acc_batch_s {
-  int device = 0;
-  std::vector<acc_batch_problem> problems;
-  int k = 0;                 // vectors per thread (tile size)
-  uint32_t total_tiles = 0;
-  void* d_desc = nullptr;
-  void* d_map = nullptr;
-  void* d_ws = nullptr;      // SyncCtl + status words
-  size_t ws_bytes = 0;
+// (workgroup size, vectors per thread) instantiations offered to the tuner
+struct BatchShape {
+  int block, k;
 };
+static const BatchShape kShapes[] = {{256, 2}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};
+static const int kRuns[] = {1, 4, 16};
 
-namespace acc {
+static const void* shape_kernel(int block, int k) {
+  if (block == 256 && k == 2) return (const void*)k_batch<256, 2>;
+  if (block == 256 && k == 4) return (const void*)k_batch<256, 4>;
+  if (block == 256 && k == 8) return (const void*)k_batch<256, 8>;
+  if (block == 512 && k == 2) return (const void*)k_batch<512, 2>;
+  if (block == 512 && k == 4) return (const void*)k_batch<512, 4>;
+  return nullptr;
+}
 
 static int type_index(acc_type t) {
   switch (t) {
@@ -200,28 +361,52 @@ static int type_index(acc_type t) {
   }
 }
 
-static void batch_free(acc_batch_t b) {
-  if (b->d_desc) dev_free(b->device, b->d_desc);
-  if (b->d_map) dev_free(b->device, b->d_map);
-  if (b->d_ws) dev_free(b->device, b->d_ws);
-  b->d_desc = b->d_map = b->d_ws = nullptr;
+int batch_validate(const acc_batch_problem* problems, int64_t count) {
+  for (int64_t i = 0; i < count; ++i) {
+    const acc_batch_problem& p = problems[i];
+    const long long li = (long long)i;
+    if (p.kind < ACC_BATCH_SCAN || p.kind > ACC_BATCH_HISTOGRAM)
+      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: kind %d", li, p.kind);
+    if (type_index((acc_type)p.type) < 0)
+      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: element type %d", li, p.type);
+    if (p.kind == ACC_BATCH_HISTOGRAM ? p.op != ACC_OP_ADD : (p.op != ACC_OP_ADD && p.op != ACC_OP_MAX))
+      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: combine %d", li, p.op);
+    if (p.n < 0 || p.m < 0) return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: negative size", li);
+    if ((p.kind == ACC_BATCH_FOLD_SEG || p.kind == ACC_BATCH_HISTOGRAM) && p.aux_type != ACC_INT32 &&
+        p.aux_type != ACC_INT64)
+      return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: offsets / indices must be Int32 or Int64", li);
+    if (p.kind == ACC_BATCH_SCAN || p.kind == ACC_BATCH_FOLD) {
+      if (p.kind == ACC_BATCH_FOLD && p.n == 0 && !p.has_seed)
+        return fail(ACC_ERROR_EMPTY_ARRAY, "batch: problem %lld: fold1 of an empty array", li);
+      if (p.n == 0)
+        return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: empty problems are served by the plain calls", li);
+      if (!p.in || !p.out || (uintptr_t)p.in % 16 || (p.kind == ACC_BATCH_SCAN && (uintptr_t)p.out % 16))
+        return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: arrays must be non-null and 16-byte aligned", li);
+    } else if (p.kind == ACC_BATCH_FOLD_SEG) {
+      if (p.m == 0) return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: no segments", li);
+      if (!p.out || !p.aux || (p.n && !p.in)) return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: null array", li);
+    } else {
+      if (p.m == 0 || p.m > HIST_MAX_BINS)
+        return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: histograms of 1..%d bins", li, HIST_MAX_BINS);
+      if (!p.out || (p.n && !p.in)) return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: null array", li);
+    }
+  }
+  return ACC_SUCCESS;
 }
 
-// Lay out tiles for vectors-per-thread k and upload descriptors, map and a
-// zeroed workspace.
-static int batch_layout(acc_batch_t b, int k) {
-  batch_free(b);
-  const size_t n = b->problems.size();
-  std::vector<BatchDesc> desc(n);
-  std::vector<uint32_t> map;
-  uint64_t words = 0;
-  uint64_t tiles = 0;
-  for (size_t i = 0; i < n; ++i) {
-    const acc_batch_problem& p = b->problems[i];
+// The layout of one batch for a (shape, run length): descriptors, the
+// run→problem map, status words.  Shared by acc_hip_batch_* and the server's
+// fused rounds.
+int batch_plan(const acc_batch_problem* problems, int64_t count, int block, int k, int R, BatchPlan* plan) {
+  plan->desc.assign(count, BatchDescBytes{});
+  plan->run_prob.clear();
+  uint64_t words = 0, runs = 0;
+  size_t lds = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const acc_batch_problem& p = problems[i];
     const size_t B = type_size((acc_type)p.type);
-    const int64_t tile = (int64_t)BATCH_BLOCK * k * (int64_t)(16 / B);
-    const int64_t nt = p.n > 0 ? (p.n + tile - 1) / tile : 0;
-    BatchDesc& d = desc[i];
+    BatchDesc d;
+    memset(&d, 0, sizeof d);
     d.cls = p.kind * 8 + type_index((acc_type)p.type) * 2 + (p.op == ACC_OP_MAX ? 1 : 0);
     d.has_seed = p.has_seed ? 1u : 0u;
     d.n = p.n;
@@ -229,69 +414,138 @@ static int batch_layout(acc_batch_t b, int k) {
     d.out = p.out;
     d.seed = p.seed;
     d.status_off = words;
-    d.ntiles = (uint32_t)nt;
-    d.tile0 = (uint32_t)tiles;
-    words += (uint64_t)nt * (B == 8 ? 2 : 1);
-    tiles += (uint64_t)nt;
-    if (tiles > 0xffffffffULL) return fail(ACC_ERROR_UNSUPPORTED, "batch: too many tiles");
-    for (int64_t t = 0; t < nt; ++t) map.push_back((uint32_t)i);
+    d.aux = p.aux;
+    d.aux64 = p.aux_type == ACC_INT64 ? 1u : 0u;
+    d.m = p.m;
+    d.fill = p.fill;
+    uint64_t nr = 0;
+    if (p.kind == ACC_BATCH_SCAN || p.kind == ACC_BATCH_FOLD) {
+      const int64_t tile = (int64_t)block * k * (int64_t)(16 / B);
+      const int64_t nt = (p.n + tile - 1) / tile;
+      d.ntiles = (uint32_t)nt;
+      // a fold reduces R consecutive tiles per run with one status slot per
+      // run; a scan is one run when it has at most R tiles, else one tile per
+      // run (one status slot per tile)
+      const int64_t r_eff = p.kind == ACC_BATCH_FOLD ? R : (nt <= R ? std::max<int64_t>(nt, 1) : 1);
+      d.R = (uint32_t)r_eff;
+      nr = (uint64_t)((nt + r_eff - 1) / r_eff);
+      words += (p.kind == ACC_BATCH_FOLD ? nr : (uint64_t)nt) * (B == 8 ? 2 : 1);
+    } else if (p.kind == ACC_BATCH_FOLD_SEG) {
+      d.ntiles = (uint32_t)p.m;
+      nr = (uint64_t)((p.m + SEG_RUN - 1) / SEG_RUN);
+    } else {
+      nr = p.n > 0 ? (uint64_t)((p.n + HIST_RUN - 1) / HIST_RUN) : 1;  // run 0 writes the initial bins
+      d.ntiles = (uint32_t)nr;
+      words += 1;
+      lds = std::max(lds, (size_t)p.m * B);
+    }
+    d.run0 = (uint32_t)runs;
+    runs += nr;
+    if (runs > 0xffffffffULL) return fail(ACC_ERROR_UNSUPPORTED, "batch: too many runs");
+    for (uint64_t r = 0; r < nr; ++r) plan->run_prob.push_back((uint32_t)i);
+    memcpy(&plan->desc[i], &d, sizeof d);
   }
-  b->k = k;
-  b->total_tiles = (uint32_t)tiles;
-  ACC_CHECK(dev_malloc(b->device, std::max<size_t>(n, 1) * sizeof(BatchDesc), &b->d_desc));
-  ACC_CHECK(dev_malloc(b->device, std::max<size_t>(map.size(), 1) * sizeof(uint32_t), &b->d_map));
-  b->ws_bytes = 256 + words * 8;
+  plan->block = block;
+  plan->k = k;
+  plan->R = R;
+  plan->total_runs = (uint32_t)runs;
+  plan->status_words = words;
+  plan->lds_bytes = lds;
+  return ACC_SUCCESS;
+}
+
+int batch_launch_plan(int device, const BatchPlan& plan, const void* d_desc, const void* d_map, void* d_ws,
+                      hipStream_t stream) {
+  if (!plan.total_runs) return ACC_SUCCESS;
+  const DeviceInfo* di;
+  ACC_CHECK(device_info(device, &di));
+  BatchArgs a{(const BatchDesc*)d_desc, (const uint32_t*)d_map, plan.total_runs, (SyncCtl*)d_ws,
+              (uint64_t*)((char*)d_ws + 256)};
+  const void* kern = shape_kernel(plan.block, plan.k);
+  if (!kern) return fail(ACC_ERROR_INVALID_VALUE, "batch: shape %d x %d", plan.block, plan.k);
+  int per_cu = blocks_per_cu(kern, plan.block);
+  if (plan.lds_bytes) {  // the histogram bins' dynamic LDS can lower the occupancy
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, plan.block, plan.lds_bytes) == hipSuccess && nb >= 1)
+      per_cu = std::min(per_cu, nb);
+    else
+      (void)hipGetLastError();
+  }
+  int64_t grid = (int64_t)di->compute_units * per_cu;
+  if (grid > (int64_t)plan.total_runs) grid = plan.total_runs;
+  void* params[] = {&a};
+  ACC_HIP_CHECK(hipLaunchKernel(kern, dim3((unsigned)grid), dim3(plan.block), params, plan.lds_bytes, stream));
+  return ACC_SUCCESS;
+}
+
+// the host fills an array of BatchDescBytes that the kernel reads as BatchDesc:
+// the strides must be equal
+static_assert(sizeof(BatchDesc) == sizeof(BatchDescBytes), "descriptor size");
+
+}  // namespace acc
+
+// ----------------------------------------------------------------- host side
+using namespace acc;
+
+struct acc_batch_s {
+  int device = 0;
+  std::vector<acc_batch_problem> problems;
+  BatchPlan plan;
+  void* d_desc = nullptr;
+  void* d_map = nullptr;
+  void* d_ws = nullptr;  // SyncCtl + status words
+  size_t ws_bytes = 0;
+};
+
+namespace acc {
+
+static void batch_free(acc_batch_t b) {
+  if (b->d_desc) dev_free(b->device, b->d_desc);
+  if (b->d_map) dev_free(b->device, b->d_map);
+  if (b->d_ws) dev_free(b->device, b->d_ws);
+  b->d_desc = b->d_map = b->d_ws = nullptr;
+}
+
+// Lay out runs for a shape and run length; upload descriptors, map and a
+// zeroed workspace.
+static int batch_layout(acc_batch_t b, int block, int k, int R) {
+  batch_free(b);
+  ACC_CHECK(batch_plan(b->problems.data(), (int64_t)b->problems.size(), block, k, R, &b->plan));
+  const size_t n = b->problems.size();
+  ACC_CHECK(dev_malloc(b->device, std::max<size_t>(n, 1) * sizeof(BatchDescBytes), &b->d_desc));
+  ACC_CHECK(dev_malloc(b->device, std::max<size_t>(b->plan.run_prob.size(), 1) * sizeof(uint32_t), &b->d_map));
+  b->ws_bytes = 256 + b->plan.status_words * 8;
   ACC_CHECK(dev_malloc(b->device, b->ws_bytes, &b->d_ws));
-  ACC_HIP_CHECK(hipMemcpy(b->d_desc, desc.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice));
-  if (!map.empty())
-    ACC_HIP_CHECK(hipMemcpy(b->d_map, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  ACC_HIP_CHECK(hipMemcpy(b->d_desc, b->plan.desc.data(), n * sizeof(BatchDescBytes), hipMemcpyHostToDevice));
+  if (!b->plan.run_prob.empty())
+    ACC_HIP_CHECK(hipMemcpy(b->d_map, b->plan.run_prob.data(), b->plan.run_prob.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice));
   ACC_HIP_CHECK(hipMemset(b->d_ws, 0, b->ws_bytes));
   // warm the occupancy cache so a later launch inside a stream capture only
   // enqueues the kernel
-  switch (k) {
-    case 1: blocks_per_cu((const void*)k_batch<BATCH_BLOCK, 1>, BATCH_BLOCK); break;
-    case 2: blocks_per_cu((const void*)k_batch<BATCH_BLOCK, 2>, BATCH_BLOCK); break;
-    default: blocks_per_cu((const void*)k_batch<BATCH_BLOCK, 4>, BATCH_BLOCK); break;
-  }
-  const DeviceInfo* di;
-  return device_info(b->device, &di);
+  blocks_per_cu(shape_kernel(block, k), block);
+  return ACC_SUCCESS;
 }
 
 static int batch_enqueue(acc_batch_t b, hipStream_t stream) {
-  if (!b->total_tiles) return ACC_SUCCESS;
-  const DeviceInfo* di;
-  ACC_CHECK(device_info(b->device, &di));
-  BatchArgs a{(const BatchDesc*)b->d_desc, (const uint32_t*)b->d_map, b->total_tiles, (SyncCtl*)b->d_ws,
-              (uint64_t*)((char*)b->d_ws + 256)};
-  auto launch = [&](auto kern) -> int {
-    int64_t grid = (int64_t)di->compute_units * blocks_per_cu((const void*)kern, BATCH_BLOCK);
-    if (grid > (int64_t)b->total_tiles) grid = b->total_tiles;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(BATCH_BLOCK), 0, stream, a);
-    ACC_HIP_CHECK(hipGetLastError());
-    return ACC_SUCCESS;
-  };
-  switch (b->k) {
-    case 1: return launch(k_batch<BATCH_BLOCK, 1>);
-    case 2: return launch(k_batch<BATCH_BLOCK, 2>);
-    case 4: return launch(k_batch<BATCH_BLOCK, 4>);
-  }
-  return fail(ACC_ERROR_INVALID_VALUE, "batch: tile size %d", b->k);
+  return batch_launch_plan(b->device, b->plan, b->d_desc, b->d_map, b->d_ws, stream);
 }
 
-// Signature of a batch for the tuning cache: problem count, element-size
-// histogram (log2 buckets) and class mix.
+// Signature of a batch for the tuning cache: problem count, size histogram
+// (log2 buckets) and class mix.
 static std::string batch_signature(const acc_batch_t b) {
   int hist[64] = {0};
-  int cls[16] = {0};
+  int cls[32] = {0};
   for (auto& p : b->problems) {
     int lg = 0;
-    while (lg < 63 && ((int64_t)1 << (lg + 1)) <= p.n) ++lg;
+    while (lg < 62 && ((int64_t)1 << (lg + 1)) <= p.n) ++lg;
     hist[lg / 2]++;
-    cls[(p.kind * 8 + type_index((acc_type)p.type) * 2 + (p.op == ACC_OP_MAX)) & 15]++;
+    cls[(p.kind * 8 + type_index((acc_type)p.type) * 2 + (p.op == ACC_OP_MAX)) & 31]++;
   }
   std::string s = "n" + std::to_string(b->problems.size());
-  for (int i = 0; i < 32; ++i) if (hist[i]) s += "_h" + std::to_string(i) + ":" + std::to_string(hist[i] * 20 / (int)b->problems.size());
-  for (int i = 0; i < 16; ++i) if (cls[i]) s += "_c" + std::to_string(i) + ":" + std::to_string(cls[i] * 20 / (int)b->problems.size());
+  const int np = std::max<int>(1, (int)b->problems.size());
+  for (int i = 0; i < 32; ++i) if (hist[i]) s += "_h" + std::to_string(i) + ":" + std::to_string(hist[i] * 20 / np);
+  for (int i = 0; i < 32; ++i) if (cls[i]) s += "_c" + std::to_string(i) + ":" + std::to_string(cls[i] * 20 / np);
   return s;
 }
 
@@ -304,32 +558,36 @@ static std::string tune_path() {
   return dir + "/accelerate-hip-tune.txt";
 }
 
-static int tune_lookup(const std::string& sig) {
+static bool tune_lookup(const std::string& sig, int* block, int* k, int* R) {
   FILE* f = fopen(tune_path().c_str(), "r");
-  if (!f) return 0;
+  if (!f) return false;
   char line[1024];
-  int k = 0;
+  bool found = false;
   while (fgets(line, sizeof line, f)) {
     char key[900];
-    int v;
-    if (sscanf(line, "batch %899s %d", key, &v) == 2 && sig == key) k = v;
+    int b, kk, r;
+    if (sscanf(line, "batch2 %899s %d %d %d", key, &b, &kk, &r) == 4 && sig == key && shape_kernel(b, kk) && r >= 1) {
+      *block = b, *k = kk, *R = r;
+      found = true;
+    }
   }
   fclose(f);
-  return k;
+  return found;
 }
 
-static void tune_store(const std::string& sig, int k) {
+static void tune_store(const std::string& sig, int block, int k, int R) {
   FILE* f = fopen(tune_path().c_str(), "a");
   if (!f) return;
-  fprintf(f, "batch %s %d\n", sig.c_str(), k);
+  fprintf(f, "batch2 %s %d %d %d\n", sig.c_str(), block, k, R);
   fclose(f);
 }
 
-// Time each candidate tile size on the batch itself; keep the fastest.
+// Time each candidate (workgroup size, vectors per thread, run length) on the
+// batch itself; keep the fastest.
 static int batch_autotune(acc_batch_t b) {
   const std::string sig = batch_signature(b);
-  int k = tune_lookup(sig);
-  if (k == 1 || k == 2 || k == 4) return batch_layout(b, k);
+  int bb, kk, rr;
+  if (tune_lookup(sig, &bb, &kk, &rr)) return batch_layout(b, bb, kk, rr);
   // the timing runs read the inputs and write the outputs: order them after
   // everything already queued on the device (inputs produced on any stream)
   ACC_HIP_CHECK(hipDeviceSynchronize());
@@ -339,20 +597,26 @@ static int batch_autotune(acc_batch_t b) {
   ACC_HIP_CHECK(hipEventCreate(&e0));
   ACC_HIP_CHECK(hipEventCreate(&e1));
   float best = 1e30f;
-  int best_k = 2;
-  for (int cand : kBatchK) {
-    int rc = batch_layout(b, cand);
-    if (!rc) rc = batch_enqueue(b, st);  // warm-up
-    if (!rc) {
-      (void)hipEventRecord(e0, st);
-      for (int r = 0; r < 3 && !rc; ++r) rc = batch_enqueue(b, st);
-      (void)hipEventRecord(e1, st);
-      (void)hipEventSynchronize(e1);
-    }
-    float ms = 0;
-    if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
-      best = ms;
-      best_k = cand;
+  BatchShape best_s = {256, 4};
+  int best_r = 4;
+  int rc = ACC_SUCCESS;
+  for (const BatchShape& sh : kShapes) {
+    for (int R : kRuns) {
+      rc = batch_layout(b, sh.block, sh.k, R);
+      if (!rc) rc = batch_enqueue(b, st);  // warm-up
+      if (!rc) {
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 3 && !rc; ++r) rc = batch_enqueue(b, st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+      }
+      float ms = 0;
+      if (!rc && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+        best = ms;
+        best_s = sh;
+        best_r = R;
+      }
+      if (rc) break;
     }
     if (rc) break;
   }
@@ -360,8 +624,9 @@ static int batch_autotune(acc_batch_t b) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(st);
-  tune_store(sig, best_k);
-  return batch_layout(b, best_k);
+  if (rc) return rc;
+  tune_store(sig, best_s.block, best_s.k, best_r);
+  return batch_layout(b, best_s.block, best_s.k, best_r);
 }
 
 }  // namespace acc
@@ -370,31 +635,14 @@ extern "C" {
 
 int acc_hip_batch_create(int device, const acc_batch_problem* problems, int64_t count, int tile, acc_batch_t* out) {
   if (!out || (count > 0 && !problems) || count < 0) return fail(ACC_ERROR_INVALID_VALUE, "batch: bad arguments");
-  if (tile != 0 && tile != 1 && tile != 2 && tile != 4)
-    return fail(ACC_ERROR_INVALID_VALUE, "batch: tile must be 0 (autotune), 1, 2 or 4");
-  for (int64_t i = 0; i < count; ++i) {
-    const acc_batch_problem& p = problems[i];
-    if (p.kind != ACC_BATCH_SCAN && p.kind != ACC_BATCH_FOLD)
-      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: kind %d", (long long)i, p.kind);
-    if (type_index((acc_type)p.type) < 0)
-      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: element type %d", (long long)i, p.type);
-    if (p.op != ACC_OP_ADD && p.op != ACC_OP_MAX)
-      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: combine %d", (long long)i, p.op);
-    if (p.n < 0) return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: negative size", (long long)i);
-    if (p.kind == ACC_BATCH_FOLD && p.n == 0 && !p.has_seed)
-      return fail(ACC_ERROR_EMPTY_ARRAY, "batch: problem %lld: fold1 of an empty array", (long long)i);
-    if (p.n > 0 && (!p.in || !p.out || (uintptr_t)p.in % 16 || (p.kind == ACC_BATCH_SCAN && (uintptr_t)p.out % 16)))
-      return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: arrays must be non-null and 16-byte aligned",
-                  (long long)i);
-    if (p.n == 0)
-      return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: empty problems are served by the plain calls",
-                  (long long)i);
-  }
+  if (tile != 0 && tile != 2 && tile != 4 && tile != 8)
+    return fail(ACC_ERROR_INVALID_VALUE, "batch: tile must be 0 (autotune), 2, 4 or 8");
+  ACC_CHECK(batch_validate(problems, count));
   ACC_CHECK(set_device(device));
   auto b = new acc_batch_s();
   b->device = device;
   b->problems.assign(problems, problems + count);
-  int rc = tile ? batch_layout(b, tile) : batch_autotune(b);
+  int rc = tile ? batch_layout(b, 256, tile, 4) : batch_autotune(b);
   if (rc) {
     batch_free(b);
     delete b;
@@ -411,10 +659,45 @@ int acc_hip_batch_launch(acc_batch_t b, acc_stream_t s) {
   return batch_enqueue(b, s->stream);
 }
 
+int acc_hip_batch_launch_plain(acc_batch_t b, acc_stream_t s) {
+  if (!b || !s) return fail(ACC_ERROR_INVALID_VALUE, "batch: null argument");
+  if (s->device != b->device) return fail(ACC_ERROR_INVALID_VALUE, "batch: stream on another device");
+  for (const acc_batch_problem& p : b->problems) {
+    const void* seed = p.has_seed ? (const void*)&p.seed : nullptr;
+    const acc_type t = (acc_type)p.type;
+    switch (p.kind) {
+      case ACC_BATCH_SCAN:
+        ACC_CHECK(acc_hip_scan(s, t, (acc_op)p.op, ACC_LEFT_TO_RIGHT, seed, p.in, 1, p.n, p.out));
+        break;
+      case ACC_BATCH_FOLD:
+        ACC_CHECK(acc_hip_fold(s, t, (acc_op)p.op, seed, p.in, 1, p.n, p.out));
+        break;
+      case ACC_BATCH_FOLD_SEG:
+        ACC_CHECK(acc_hip_fold_seg(s, t, (acc_op)p.op, seed, p.in, 1, p.n, (acc_type)p.aux_type, p.aux, p.m, p.out));
+        break;
+      default:
+        ACC_CHECK(acc_hip_fill_async(s, p.out, &p.seed, type_size(t), (size_t)p.m));
+        ACC_CHECK(acc_hip_permute(s, t, ACC_OP_ADD, p.out, p.out, p.m, nullptr, &p.fill, (acc_type)p.aux_type, p.in,
+                                  p.n));
+        break;
+    }
+  }
+  return ACC_SUCCESS;
+}
+
 int acc_hip_batch_tile(acc_batch_t b, int* tile, uint32_t* total_tiles) {
   if (!b) return fail(ACC_ERROR_INVALID_VALUE, "batch: null");
-  if (tile) *tile = b->k;
-  if (total_tiles) *total_tiles = b->total_tiles;
+  if (tile) *tile = b->plan.k;
+  if (total_tiles) *total_tiles = b->plan.total_runs;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_batch_shape(acc_batch_t b, int* block, int* vectors, int* run, uint32_t* tickets) {
+  if (!b) return fail(ACC_ERROR_INVALID_VALUE, "batch: null");
+  if (block) *block = b->plan.block;
+  if (vectors) *vectors = b->plan.k;
+  if (run) *run = b->plan.R;
+  if (tickets) *tickets = b->plan.total_runs;
   return ACC_SUCCESS;
 }
 

@@ -499,42 +499,69 @@ int acc_hip_table_stats(acc_table_t t, acc_table_stats* st);
 int acc_hip_table_destroy(acc_table_t t);
 
 /* ================================================================ batched
- * Many independent scans and folds of uneven sizes in ONE persistent launch
- * (no counterpart in the reference, whose per-call path is
- * PTX/Execute.hs:835-881).  A batch is built once from problem descriptors
- * (device pointers; inputs 16-byte aligned, scan outputs too) and can be
- * launched many times, directly or inside a captured graph.  One launch of a
- * batch may be in flight at a time.  Problem semantics:
- *   ACC_BATCH_SCAN: has_seed == 0 -> scanl1 (n outputs);
- *                   has_seed != 0 -> scanl seed (n + 1 outputs, total last)
- *   ACC_BATCH_FOLD: out[0] = seed (+) fold of in (fold1 when has_seed == 0)
- * Element types INT32, INT64, FLOAT32, FLOAT64; combine ADD or MAX. */
-typedef enum acc_batch_kind { ACC_BATCH_SCAN = 0, ACC_BATCH_FOLD = 1 } acc_batch_kind;
+ * Many independent scans, folds, segmented folds and histograms of uneven
+ * sizes in ONE persistent launch (no counterpart in the reference, whose
+ * per-call path is PTX/Execute.hs:835-881).  A batch is built once from
+ * problem descriptors (device pointers) and can be launched many times,
+ * directly or inside a captured graph.  One launch of a batch may be in flight
+ * at a time.  Problem semantics:
+ *   ACC_BATCH_SCAN:      has_seed == 0 -> scanl1 (n outputs);
+ *                        has_seed != 0 -> scanl seed (n + 1 outputs, total last)
+ *   ACC_BATCH_FOLD:      out[0] = seed (+) fold of in (fold1 when has_seed == 0)
+ *   ACC_BATCH_FOLD_SEG:  out[s] = seed (+) fold of in[aux[s] .. aux[s+1]) for
+ *                        s < m (fold1Seg when has_seed == 0: empty segments are
+ *                        left as they are); aux: m + 1 offsets of aux_type
+ *   ACC_BATCH_HISTOGRAM: permute (+) of a constant source: out = m bins, each
+ *                        first `seed`, then out[in[i]] (+)= fill for i < n with
+ *                        0 <= in[i] < m (in: indices of aux_type); m <= 4096
+ * Element types INT32, INT64, FLOAT32, FLOAT64; combine ADD or MAX (ADD for
+ * histograms).  Scan and fold inputs (and scan outputs) must be 16-byte
+ * aligned. */
+typedef enum acc_batch_kind {
+  ACC_BATCH_SCAN = 0,
+  ACC_BATCH_FOLD = 1,
+  ACC_BATCH_FOLD_SEG = 2,
+  ACC_BATCH_HISTOGRAM = 3
+} acc_batch_kind;
 
 typedef struct acc_batch_problem {
   int32_t kind;     /* acc_batch_kind */
   int32_t type;     /* acc_type */
   int32_t op;       /* acc_op */
   int32_t has_seed;
-  int64_t n;        /* elements (> 0) */
-  const void* in;
+  int64_t n;        /* elements (scan, fold, foldSeg values), indices (histogram) */
+  const void* in;   /* values; histogram: the indices */
   void* out;
-  uint64_t seed;    /* bit pattern of the seed in the low bytes */
+  uint64_t seed;    /* bit pattern of the seed in the low bytes (histogram: the initial bin value) */
+  const void* aux;  /* foldSeg: offsets */
+  int64_t m;        /* foldSeg: segments; histogram: bins */
+  int32_t aux_type; /* foldSeg offsets, histogram indices: ACC_INT32 or ACC_INT64 */
+  int32_t reserved;
+  uint64_t fill;    /* histogram: the constant source value (bit pattern) */
 } acc_batch_problem;
 
 typedef struct acc_batch_s* acc_batch_t;
 
-/* tile = vectors per thread (1, 2 or 4); 0 lets the autotuner time the
- * candidates on this batch and remember the winner (file named by
- * $ACC_HIP_TUNE_CACHE, else ~/.cache/accelerate-hip-tune.txt).  Tuning RUNS
- * the batch several times (a cached choice does not): it first waits for all
- * work queued on the device, so inputs written by earlier calls on any stream
- * are complete, and it writes every problem's output, so the outputs must not
- * be in use by other work while acc_hip_batch_create runs. */
+/* tile = vectors per thread (2, 4 or 8, with 256-thread workgroups and runs of
+ * 4 tiles); 0 lets the autotuner time the candidates (workgroup size 256 or
+ * 512, vectors per thread, tiles per ticket) on this batch and remember the
+ * winner (file named by $ACC_HIP_TUNE_CACHE, else
+ * ~/.cache/accelerate-hip-tune.txt).  Tuning RUNS the batch several times (a
+ * cached choice does not): it first waits for all work queued on the device,
+ * so inputs written by earlier calls on any stream are complete, and it writes
+ * every problem's output, so the outputs must not be in use by other work while
+ * acc_hip_batch_create runs. */
 int acc_hip_batch_create(int device, const acc_batch_problem* problems, int64_t count, int tile,
                          acc_batch_t* b);
 int acc_hip_batch_launch(acc_batch_t b, acc_stream_t s);
+/* The same problems through the plain entry points, one call (or two for a
+ * histogram: fill, permute) per problem, in order on s: the per-call path the
+ * batch replaces, for comparison. */
+int acc_hip_batch_launch_plain(acc_batch_t b, acc_stream_t s);
+/* vectors per thread and the number of tickets (runs of work) of one launch */
 int acc_hip_batch_tile(acc_batch_t b, int* tile, uint32_t* total_tiles);
+/* the chosen shape: workgroup size, vectors per thread, tiles per ticket, tickets */
+int acc_hip_batch_shape(acc_batch_t b, int* block, int* vectors, int* run, uint32_t* tickets);
 int acc_hip_batch_destroy(acc_batch_t b);
 
 /* ================================================================ serving

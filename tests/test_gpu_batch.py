@@ -1,8 +1,10 @@
-"""Batched execution: many uneven scans/folds in one persistent launch.
+"""Batched execution: many uneven scans, folds, segmented folds and
+histograms in one persistent launch.
 
 Each problem's result must equal its standalone result (integers bit-exact,
-floating sums within the FP32 bound of the FP64 truth), for every tile size,
-for the autotuned batch and for replays of a captured graph.
+floating sums within the FP32 bound of the FP64 truth), for every shape
+(workgroup size x vectors per thread x tiles per ticket), for the autotuned
+batch and for replays of a captured graph.
 """
 import math
 
@@ -18,23 +20,39 @@ ADD, MAX = A.ADD, A.MAX
 TYPES = [np.int32, np.int64, np.float32, np.float64]
 
 
-def make_problems(stream, count, seed, max_log2=18):
+def make_problems(stream, count, seed, max_log2=18, kinds=(A.BATCH_SCAN, A.BATCH_FOLD)):
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(0, math.log(2 ** max_log2), size=count)).astype(np.int64) + 1
     probs = []
     for i, n in enumerate(sizes):
         dt = TYPES[rng.integers(0, 4)]
-        kind = A.BATCH_SCAN if rng.integers(0, 2) == 0 else A.BATCH_FOLD
-        op = ADD if rng.integers(0, 3) else MAX
+        kind = kinds[rng.integers(0, len(kinds))]
+        op = ADD if (rng.integers(0, 3) or kind == A.BATCH_HISTOGRAM) else MAX
         seed_v = None if rng.integers(0, 2) == 0 else dt(rng.integers(-5, 5))
         if np.dtype(dt).kind == "f":
             x = (rng.integers(-64, 64, size=n) / 16).astype(dt) if rng.integers(0, 2) else rng.standard_normal(n).astype(dt)
         else:
             x = rng.integers(-1000, 1000, size=n).astype(dt)
-        src = A.DeviceArray.from_numpy(stream, x)
-        on = 1 if kind == A.BATCH_FOLD else (n + 1 if seed_v is not None else n)
-        out = A.DeviceArray(on, dt)
-        probs.append(dict(kind=kind, op=op, seed=seed_v, x=x, src=src, out=out, n=int(n), dt=dt))
+        p = dict(kind=kind, op=op, seed=seed_v, x=x, n=int(n), dt=dt, aux=None, m=0, fill=0)
+        if kind == A.BATCH_FOLD_SEG:
+            m = int(rng.integers(1, max(2, n // 8)))
+            lens = rng.multinomial(int(n), np.ones(m) / m)  # some segments empty
+            off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64 if rng.integers(0, 2) else np.int32)
+            p.update(aux=A.DeviceArray.from_numpy(stream, off), off=off, m=m)
+            on = m
+        elif kind == A.BATCH_HISTOGRAM:
+            m = int(rng.integers(1, 4097))
+            idx = rng.integers(-10, m + 10, size=n).astype(np.int32 if rng.integers(0, 2) else np.int64)
+            p.update(x=idx, m=m, fill=dt(rng.integers(1, 4)), seed=dt(rng.integers(-5, 5)))
+            on = m
+        else:
+            on = 1 if kind == A.BATCH_FOLD else (n + 1 if seed_v is not None else n)
+        p["src"] = A.DeviceArray.from_numpy(stream, p["x"])
+        p["out"] = A.DeviceArray(on, dt)
+        if kind == A.BATCH_FOLD_SEG:  # fold1Seg leaves empty segments: start from a known value
+            p["out"] = A.DeviceArray.from_numpy(stream, np.full(on, 7, dt))
+            p["init"] = 7
+        probs.append(p)
     return probs
 
 
@@ -42,13 +60,33 @@ def expected(p):
     x, op, seed = p["x"], p["op"], p["seed"]
     if p["kind"] == A.BATCH_FOLD:
         return O.fold(op, seed, x)
+    if p["kind"] == A.BATCH_FOLD_SEG:
+        want = O.fold_seg(op, seed, x, p["off"].astype(np.int64))
+        if seed is None:
+            want = np.where(np.diff(p["off"].astype(np.int64)) == 0, p["dt"](p["init"]), want)
+        return want
+    if p["kind"] == A.BATCH_HISTOGRAM:
+        m = p["m"]
+        keep = (x >= 0) & (x < m)
+        return (p["seed"] + np.bincount(x[keep], minlength=m).astype(np.float64) * float(p["fill"])).astype(p["dt"])
     return O.scan(op, 0, seed, x)
 
 
 def check(stream, p, got):
     want = expected(p)
-    if np.dtype(p["dt"]).kind != "f" or p["op"] == MAX:
+    if np.dtype(p["dt"]).kind != "f" or p["op"] == MAX or p["kind"] == A.BATCH_HISTOGRAM:
+        # histogram bins of small integer counts x fill are exact in floating point too
         assert np.array_equal(got, want), (p["kind"], p["dt"], p["op"], p["n"])
+        return
+    if p["kind"] == A.BATCH_FOLD_SEG:
+        off = p["off"].astype(np.int64)
+        truth, ab = O.fold_seg_f64(p["x"], off, None if p["seed"] is None else float(p["seed"]))
+        lens = np.diff(off)
+        bound = 1e-5 * np.maximum(1, np.log2(np.maximum(lens, 2))) * ab + 1e-9
+        ok = np.abs(got.astype(np.float64) - truth) <= bound
+        if p["seed"] is None:
+            ok |= lens == 0
+        assert np.all(ok), (p["kind"], p["dt"], p["n"])
         return
     # floating (+): normwise bound against the FP64 truth
     x = p["x"].astype(np.float64)
@@ -68,12 +106,15 @@ def check(stream, p, got):
 
 
 def batch_of(probs):
-    return [(p["kind"], p["op"], p["seed"], p["src"], p["n"], p["out"]) for p in probs]
+    return [(p["kind"], p["op"], p["seed"], p["src"], p["n"], p["out"], p["aux"], p["m"], p["fill"]) for p in probs]
 
 
-@pytest.mark.parametrize("tile", [1, 2, 4])
+ALL_KINDS = (A.BATCH_SCAN, A.BATCH_FOLD, A.BATCH_FOLD_SEG, A.BATCH_HISTOGRAM)
+
+
+@pytest.mark.parametrize("tile", [2, 4, 8])
 def test_batch_matches_oracle(stream, tile):
-    probs = make_problems(stream, 150, seed=tile)
+    probs = make_problems(stream, 150, seed=tile, kinds=ALL_KINDS)
     b = A.Batch(0, batch_of(probs), tile=tile)
     assert b.tile()[0] == tile
     b.launch(stream)
@@ -83,37 +124,36 @@ def test_batch_matches_oracle(stream, tile):
 
 
 def test_batch_same_alone_and_in_mix(stream):
-    probs = make_problems(stream, 60, seed=11)
-    b = A.Batch(0, batch_of(probs), tile=2)
+    """Every problem's batched result against the same problem through the plain
+    C calls (acc_hip_batch_launch_plain): bit-exact where the arithmetic is."""
+    probs = make_problems(stream, 80, seed=11, kinds=ALL_KINDS)
+    b = A.Batch(0, batch_of(probs), tile=4)
     b.launch(stream)
     stream.synchronize()
-    for p in probs:
-        mixed = p["out"].to_numpy(stream)
-        alone = A.DeviceArray(p["out"].count, p["dt"])
-        if p["kind"] == A.BATCH_FOLD:
-            A.fold(stream, p["op"], p["seed"], p["src"], 1, p["n"], alone)
-        else:
-            A.scan(stream, p["op"], A.LEFT_TO_RIGHT, p["seed"], p["src"], 1, p["n"], alone)
-        alone = alone.to_numpy(stream)
-        if np.dtype(p["dt"]).kind != "f" or p["op"] == MAX:
-            assert np.array_equal(mixed, alone)
-        check(stream, p, mixed)
+    mixed = [p["out"].to_numpy(stream) for p in probs]
+    b.launch_plain(stream)
+    stream.synchronize()
+    for p, mx in zip(probs, mixed):
+        alone = p["out"].to_numpy(stream)
+        if np.dtype(p["dt"]).kind != "f" or p["op"] == MAX or p["kind"] == A.BATCH_HISTOGRAM:
+            assert np.array_equal(mx, alone), (p["kind"], p["dt"], p["n"])
+        check(stream, p, mx)
 
 
 def test_batch_autotuned(stream, tmp_path, monkeypatch):
     monkeypatch.setenv("ACC_HIP_TUNE_CACHE", str(tmp_path / "tune.txt"))
-    probs = make_problems(stream, 80, seed=5)
+    probs = make_problems(stream, 80, seed=5, kinds=ALL_KINDS)
     b = A.Batch(0, batch_of(probs), tile=0)
-    k, tiles = b.tile()
-    assert k in (1, 2, 4) and tiles > 0
-    assert (tmp_path / "tune.txt").read_text().strip().endswith(str(k))
+    block, k, run, tickets = b.shape()
+    assert block in (256, 512) and k in (2, 4, 8) and run in (1, 4, 16) and tickets > 0
+    assert (tmp_path / "tune.txt").read_text().strip().endswith(f"{block} {k} {run}")
     b.launch(stream)
     stream.synchronize()
     for p in probs:
         check(stream, p, p["out"].to_numpy(stream))
     # a second batch with the same signature reuses the stored choice
     b2 = A.Batch(0, batch_of(probs), tile=0)
-    assert b2.tile()[0] == k
+    assert b2.shape() == (block, k, run, tickets)
 
 
 def test_batch_graph_replays(stream):
@@ -139,6 +179,8 @@ def test_batch_rejects_bad_problems(stream):
     x = A.DeviceArray.from_numpy(stream, np.arange(10, dtype=np.int32))
     o = A.DeviceArray(10, np.int32)
     with pytest.raises(A.AccError):
-        A.Batch(0, [(A.BATCH_SCAN, A.MUL, None, x, 10, o)], tile=1)
+        A.Batch(0, [(A.BATCH_SCAN, A.MUL, None, x, 10, o)], tile=2)
     with pytest.raises(A.AccError):
-        A.Batch(0, [(A.BATCH_SCAN, ADD, None, x.view(1, 9), 9, o)], tile=1)  # misaligned input
+        A.Batch(0, [(A.BATCH_SCAN, ADD, None, x.view(1, 9), 9, o)], tile=2)  # misaligned input
+    with pytest.raises(A.AccError):
+        A.Batch(0, [(A.BATCH_HISTOGRAM, ADD, 0, x, 10, o, None, 5000, 1)], tile=2)  # too many bins

@@ -221,10 +221,12 @@ def test_c4_all_batched_problems(stream):
         probs.append((A.BATCH_FOLD, ADD, np.int32(0), d_xf.view(int(f_off[j]), k), k, d_yf.view(j * 64, 1)))
     say(f"C4 batch of {P} problems, {int(sizes.sum())} elements")
     b = A.Batch(0, probs, tile=0)
+    say(f"C4 shape (block, vectors, run, tickets) = {b.shape()}")
     g = A.Graph.capture(stream, lambda: b.launch(stream))
     for rep in range(2):
         g.launch(stream)
         stream.synchronize()
+        say(f"C4 replay {rep} done")
         got_f = d_yf.to_numpy(stream)[::64]
         for j, i in enumerate(fold_idx):
             k = int(sizes[i])

@@ -135,6 +135,20 @@ class ServerStats(ctypes.Structure):
         ("span_s", ctypes.c_double),
         ("arena_bytes", c_size_t),
         ("arena_used", c_size_t),
+        ("rounds", ctypes.c_uint64),
+        ("fused", ctypes.c_uint64),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("fuse", ctypes.c_int32),
+        ("round_requests", ctypes.c_int32),
+        ("round_bytes", ctypes.c_int64),
+        ("block", ctypes.c_int32),
+        ("vectors", ctypes.c_int32),
+        ("run", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 class KernelInfo(ctypes.Structure):
@@ -258,6 +272,7 @@ SIGNATURES = {
                                     POINTER(ctypes.c_uint32)]),
     "acc_hip_batch_destroy": (c_int, [c_void_p]),
     "acc_hip_server_create": (c_int, [c_int, c_int, c_size_t, POINTER(c_void_p)]),
+    "acc_hip_server_configure": (c_int, [c_void_p, POINTER(ServerConfig)]),
     "acc_hip_server_submit": (c_int, [c_void_p, POINTER(Request), c_int64, c_int, POINTER(ctypes.c_uint64)]),
     "acc_hip_server_wait": (c_int, [c_void_p, ctypes.c_uint64, POINTER(c_void_p), POINTER(ctypes.c_double)]),
     "acc_hip_server_release": (c_int, [c_void_p, ctypes.c_uint64]),
@@ -636,11 +651,19 @@ class Graph:
 class Server:
     """The serving loop (acc_hip_server_*)."""
 
-    def __init__(self, device=0, slots=4, hbm_budget=0):
+    def __init__(self, device=0, slots=4, hbm_budget=0, fuse=True, round_requests=0, round_bytes=0,
+                 shape=(0, 0, 0)):
         h = c_void_p()
         check(lib().acc_hip_server_create(device, slots, hbm_budget, ctypes.byref(h)))
         self.handle = h
         self._keep = []  # host buffers referenced by queued requests (stencil weights)
+        self.configure(fuse, round_requests, round_bytes, shape)
+
+    def configure(self, fuse=True, round_requests=0, round_bytes=0, shape=(0, 0, 0)):
+        """fuse: run requests with a batched form in fused rounds; shape:
+        (workgroup size, vectors per thread, tiles per run) of the fused launch."""
+        c = ServerConfig(1 if fuse else 0, round_requests, round_bytes, shape[0], shape[1], shape[2], 0)
+        check(lib().acc_hip_server_configure(self.handle, ctypes.byref(c)))
 
     @staticmethod
     def request(skeleton, dtype, op=ADD, seed=None, src=None, n=0, m=0, aux=None, fill=0, direction=0,

@@ -18,10 +18,13 @@
 //                publishing its inclusive prefix
 //                Each problem owns a slice of the status words, so no
 //                look-back crosses into another.
-//   foldSeg      a run is 64 segments; each wave folds 16 of them in order
-//                (the reference's warp per segment, PTX/CodeGen/FoldSeg.hs:274-458)
-//   histogram    a run is 64 Ki indices folded into a private LDS copy of the
-//                bins, then flushed with one global atomic per non-zero bin;
+//   foldSeg      a run is 32 consecutive segments, their elements cut into
+//                one-vector-per-lane chunks dealt to the waves (the reference
+//                uses a warp per segment, PTX/CodeGen/FoldSeg.hs:274-458,
+//                which leaves a wave one load in flight on short segments)
+//   histogram    a run is 128 Ki index positions folded into a private LDS
+//                copy of the bins, then flushed with one global atomic per
+//                non-zero bin;
 //                the problem's first run writes the initial bins and
 //                publishes a tagged flag, which every run waits for before
 //                it flushes (the first run is ticketed before the others, so
@@ -73,8 +76,8 @@ struct BatchArgs {
   uint64_t* status;
 };
 
-constexpr int SEG_RUN = 64;          // segments per foldSeg run
-constexpr int64_t HIST_RUN = 65536;  // indices per histogram run
+constexpr int SEG_RUN = 32;          // segments per foldSeg run
+constexpr int64_t HIST_RUN = 131072; // index positions per histogram run
 constexpr int HIST_MAX_BINS = 4096;  // bins held in LDS (8-byte bins: 32 KiB)
 
 template <typename T> __device__ __forceinline__ T bits_to(uint64_t b) {
@@ -145,7 +148,7 @@ __device__ __forceinline__ void fold_run(const BatchDesc& d, uint64_t* status, u
       T res = agg;
       if (!first) res = op(wave_lookback<T>(status, r, epoch, op), res);
       if (d.has_seed) res = op(bits_to<T>(d.seed), res);  // seed on the left
-      if (lane == 0) *(T*)d.out = res;
+      if (lane == 0) gstore((T*)d.out, res);
     }
   }
 }
@@ -183,48 +186,133 @@ __device__ __forceinline__ void scan_fold_run(const BatchDesc& d, int kind, uint
   else fold_run<T, Op, BLOCK, K>(d, status, r, epoch, (T*)s_raw);
 }
 
-// foldSeg / fold1Seg: wave w folds segments [r*SEG_RUN + w*16, +16) in order,
-// 256 elements per step (4 consecutive per lane, an in-lane fold, then the
-// in-order wave reduction); empty segments get the seed (fold1Seg: left as
-// they are).
+// foldSeg / fold1Seg.  A run is SEG_RUN consecutive segments, whose elements
+// are contiguous.  The run's elements are cut into chunks of one 16-byte
+// vector per lane (64 * VEC elements, aligned; a chunk never spans two
+// segments, so a segment's first and last chunks are partial) and the chunks
+// are dealt to the waves in blocks of SEG_U: each wave keeps SEG_U vector
+// loads in flight, whatever the segments' lengths, and a long segment is
+// shared by all the waves of the workgroup.  Per chunk: an in-lane fold of
+// the elements inside its segment (identity elsewhere), the wave reduction,
+// and a fold into the wave's partial result for that segment (LDS); at the
+// end the waves' partials are folded in wave order and written, one
+// coalesced store per run.  Empty segments get the seed (fold1Seg: left as
+// they are).  The combination order depends only on the segment and the
+// workgroup shape, so a result does not depend on the other problems.
+constexpr int SEG_U = 8;  // chunks in flight per wave
+
+struct SegTable {  // LDS: the run's segments and the waves' partial results
+  int64_t lo[SEG_RUN], hi[SEG_RUN];  // positions (relative to the aligned base)
+  int32_t pre[SEG_RUN], end[SEG_RUN];  // first chunk, one past the last chunk
+  int32_t chunks;
+  int32_t pad;
+  uint64_t part[8 * SEG_RUN];  // [wave][segment], T in the low bytes
+};
+
 template <typename T, typename Op, int BLOCK>
-__device__ __forceinline__ void fold_seg_run(const BatchDesc& d, uint32_t r) {
+__device__ __forceinline__ void fold_seg_run(const BatchDesc& d, uint32_t r, SegTable* tb) {
   constexpr int W = BLOCK / 64;
-  constexpr int PER = SEG_RUN / W;
+  static_assert(W <= 8, "partials for at most 8 waves");
+  constexpr int VEC = Vec<T>::N;
+  constexpr int64_t CH = 64 * VEC;
   Op op{};
   const int lane = lane_id(), w = threadIdx.x / 64;
-  const T* in = (const T*)d.in;
-  const int64_t s0 = (int64_t)r * SEG_RUN + w * PER;
-  for (int k = 0; k < PER; ++k) {
-    const int64_t s = s0 + k;
-    if (s >= d.m) break;  // wave-uniform
-    const int64_t lo = d.aux64 ? ((const int64_t*)d.aux)[s] : ((const int32_t*)d.aux)[s];
-    const int64_t hi = d.aux64 ? ((const int64_t*)d.aux)[s + 1] : ((const int32_t*)d.aux)[s + 1];
-    Maybe<T> acc;
-    acc.v = T();
-    acc.ok = d.has_seed;
-    if (d.has_seed) acc.v = bits_to<T>(d.seed);
-    for (int64_t i = lo; i < hi; i += 256) {
-      const int64_t e = i + lane * 4;
-      Maybe<T> x;
-      x.v = T();
-      x.ok = 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (e + j < hi) {
-          Maybe<T> y;
-          y.v = in[e + j];
-          y.ok = 1;
-          x = mcombine(op, x, y);
-        }
-      acc = mcombine(op, acc, wave_reduce_m(x, op));
+  const T id = identity<T>(op);
+  // positions are relative to the 16-byte boundary at or below `in`; the
+  // valid ones are [shift, shift + n)
+  const int shift = (int)(((uintptr_t)d.in % 16) / sizeof(T));
+  const T* base = (const T*)d.in - shift;
+  const int64_t last_vec = d.n > 0 ? (shift + d.n - 1) / VEC * VEC : 0;
+  const int64_t s0 = (int64_t)r * SEG_RUN;
+  const int ns = (int)min((int64_t)SEG_RUN, d.m - s0);
+  T* part = (T*)tb->part;
+  if (w == 0) {
+    int64_t lo = 0, hi = 0;
+    int nch = 0;
+    if (lane < ns) {
+      if (d.aux64) {
+        lo = gload((const int64_t*)d.aux + s0 + lane);
+        hi = gload((const int64_t*)d.aux + s0 + lane + 1);
+      } else {
+        lo = gload((const int32_t*)d.aux + s0 + lane);
+        hi = gload((const int32_t*)d.aux + s0 + lane + 1);
+      }
+      lo += shift;
+      hi += shift;
+      if (hi > lo && d.n > 0) nch = (int)((hi - lo / VEC * VEC + CH - 1) / CH);
     }
-    if (lane == 0 && acc.ok) ((T*)d.out)[s] = acc.v;
+    const int incl = wave_inclusive_scan(nch, OpAdd{});
+    if (lane < ns) {
+      tb->lo[lane] = lo;
+      tb->hi[lane] = hi;
+      tb->pre[lane] = incl - nch;
+      tb->end[lane] = incl;
+    }
+    if (lane == 63) tb->chunks = incl;
   }
+  for (int i = threadIdx.x; i < W * SEG_RUN; i += BLOCK) part[i] = id;
+  __syncthreads();
+  const int C = tb->chunks;
+  // lane j < ns holds the end of segment j: the segment of chunk c is the
+  // number of segments that end at or before c
+  const int end_l = lane < ns ? tb->end[lane] : 0x7fffffff;
+  for (int cb = w * SEG_U; cb < C; cb += W * SEG_U) {
+    Vec<T> v[SEG_U];
+    int64_t pos[SEG_U];
+    int js[SEG_U];
+#pragma unroll
+    for (int u = 0; u < SEG_U; ++u) {
+      const int c = cb + u;
+      js[u] = -1;
+      pos[u] = 0;
+      if (c < C) {  // wave-uniform
+        const int j = (int)__popcll(__ballot(end_l <= c));
+        js[u] = j;
+        pos[u] = tb->lo[j] / VEC * VEC + (int64_t)(c - tb->pre[j]) * CH + lane * VEC;
+        // a vector past the segment's end reads the last vector holding an
+        // element of the array (then every element of it is masked)
+        v[u] = load16_stream(base + min(max(pos[u], (int64_t)0), last_vec));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SEG_U; ++u) {
+      const int j = js[u];
+      if (j < 0) break;  // wave-uniform
+      const int64_t lo = tb->lo[j], hi = tb->hi[j];
+      T x = id;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        const int64_t q = pos[u] + e;
+        x = op(x, q >= lo && q < hi ? v[u].e[e] : id);
+      }
+      const T rr = wave_reduce(x, op);
+      if (lane == 0) part[w * SEG_RUN + j] = op(part[w * SEG_RUN + j], rr);
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ns) {
+    const int j = threadIdx.x;
+    T* out = (T*)d.out + s0;
+    if (tb->end[j] == tb->pre[j]) {
+      if (d.has_seed) gstore(out + j, bits_to<T>(d.seed));
+    } else {
+      T res = part[j];
+#pragma unroll
+      for (int k = 1; k < W; ++k) res = op(res, part[k * SEG_RUN + j]);
+      gstore(out + j, d.has_seed ? op(bits_to<T>(d.seed), res) : res);  // seed on the left
+    }
+  }
+  __syncthreads();  // the table is rebuilt by the next run
 }
 
 // The histogram: out[idx[i]] = out[idx[i]] (+) fill for 0 <= idx[i] < m, with
 // out initially `seed` in every bin (permute (+) of a constant source).
+// LDS float32 atomic adds run ~20x slower than integer or float64 ones on
+// gfx950 (measured), so float32 bins accumulate in float64 and are rounded
+// once, at the flush.
+template <typename T> struct HistAcc { using type = T; };
+template <> struct HistAcc<float> { using type = double; };
+
 template <typename T>
 __device__ __forceinline__ void lds_add(T* p, T v) {
   if constexpr (std::is_floating_point<T>::value) atomicAdd(p, v);
@@ -232,33 +320,58 @@ __device__ __forceinline__ void lds_add(T* p, T v) {
 }
 template <typename T>
 __device__ __forceinline__ void global_add(T* p, T v) {
-  if constexpr (std::is_floating_point<T>::value) unsafeAtomicAdd(p, v);
-  else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  typedef __attribute__((address_space(1))) T gT;
+  if constexpr (std::is_same<T, float>::value) (void)__builtin_amdgcn_global_atomic_fadd_f32((gT*)p, v);
+  else if constexpr (std::is_same<T, double>::value) (void)__builtin_amdgcn_global_atomic_fadd_f64((gT*)p, v);
+  else __hip_atomic_fetch_add((gT*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// A run is HIST_RUN index positions (relative to the 16-byte boundary at or
+// below the indices; the valid ones are [shift, shift + n)), read as 16-byte
+// vectors, HIST_U per thread in flight, and folded into a private LDS copy of
+// the bins.
+constexpr int HIST_U = 8;
 
 template <typename T, typename I, int BLOCK>
 __device__ __forceinline__ void hist_run(const BatchDesc& d, uint64_t* status, uint32_t r, uint32_t epoch,
-                                         T* bins) {
+                                         unsigned char* s_bins) {
+  using L = typename HistAcc<T>::type;
+  L* bins = (L*)s_bins;
+  constexpr int IV = Vec<I>::N;
   T* out = (T*)d.out;
   const int64_t m = d.m;
-  const T fill = bits_to<T>(d.fill);
+  const L fill = (L)bits_to<T>(d.fill);
   const uint32_t ready = make_tag(epoch, ST_PREFIX);
   if (r == 0) {
     // the first run initialises the bins, makes the stores visible to every
     // XCD, then publishes the flag
     const T z = bits_to<T>(d.seed);
-    for (int64_t b = threadIdx.x; b < m; b += BLOCK) out[b] = z;
+    for (int64_t b = threadIdx.x; b < m; b += BLOCK) gstore(out + b, z);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(status, (uint64_t)ready << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  for (int64_t b = threadIdx.x; b < m; b += BLOCK) bins[b] = T(0);
+  for (int64_t b = threadIdx.x; b < m; b += BLOCK) bins[b] = L(0);
   __syncthreads();
-  const I* idx = (const I*)d.in;
-  const int64_t i0 = (int64_t)r * HIST_RUN, i1 = min(d.n, i0 + HIST_RUN);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-    const int64_t j = (int64_t)idx[i];
-    if (j >= 0 && j < m) lds_add(&bins[j], fill);
+  const int shift = (int)(((uintptr_t)d.in % 16) / sizeof(I));
+  const I* base = (const I*)d.in - shift;
+  const int64_t v_lo = shift, v_hi = shift + d.n;  // valid positions
+  const int64_t last_vec = d.n > 0 ? (v_hi - 1) / IV * IV : 0;
+  const int64_t p0 = (int64_t)r * HIST_RUN, p1 = d.n > 0 ? min(v_hi, p0 + HIST_RUN) : 0;
+  constexpr int64_t STEP = (int64_t)BLOCK * IV * HIST_U;
+  for (int64_t q0 = p0 + (int64_t)threadIdx.x * IV; q0 < p1; q0 += STEP) {
+    Vec<I> v[HIST_U];
+#pragma unroll
+    for (int u = 0; u < HIST_U; ++u) v[u] = load16_stream(base + min(q0 + (int64_t)u * BLOCK * IV, last_vec));
+#pragma unroll
+    for (int u = 0; u < HIST_U; ++u) {
+      const int64_t q = q0 + (int64_t)u * BLOCK * IV;
+#pragma unroll
+      for (int e = 0; e < IV; ++e) {
+        const int64_t j = (int64_t)v[u].e[e];
+        if (q + e >= v_lo && q + e < p1 && j >= 0 && j < m) lds_add(&bins[j], fill);
+      }
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0)
@@ -266,15 +379,15 @@ __device__ __forceinline__ void hist_run(const BatchDesc& d, uint64_t* status, u
       __builtin_amdgcn_s_sleep(2);
   __syncthreads();
   for (int64_t b = threadIdx.x; b < m; b += BLOCK) {
-    const T v = bins[b];
-    if (v != T(0)) global_add(&out[b], v);
+    const L v = bins[b];
+    if (v != L(0)) global_add(&out[b], (T)v);
   }
   __syncthreads();
 }
 
 template <int BLOCK, int K>
 __device__ __forceinline__ void batch_run(const BatchDesc& d, uint64_t* st, uint32_t r, uint32_t epoch,
-                                          unsigned char* s_raw, unsigned char* s_dyn) {
+                                          unsigned char* s_raw, unsigned char* s_dyn, SegTable* s_seg) {
   const int kind = d.cls >> 3;
   const int c = d.cls & 7;
   if (kind <= BK_FOLD) {
@@ -290,26 +403,26 @@ __device__ __forceinline__ void batch_run(const BatchDesc& d, uint64_t* st, uint
     }
   } else if (kind == BK_FOLD_SEG) {
     switch (c) {
-      case 0: fold_seg_run<int32_t, OpAdd, BLOCK>(d, r); break;
-      case 1: fold_seg_run<int32_t, OpMax, BLOCK>(d, r); break;
-      case 2: fold_seg_run<int64_t, OpAdd, BLOCK>(d, r); break;
-      case 3: fold_seg_run<int64_t, OpMax, BLOCK>(d, r); break;
-      case 4: fold_seg_run<float, OpAdd, BLOCK>(d, r); break;
-      case 5: fold_seg_run<float, OpMax, BLOCK>(d, r); break;
-      case 6: fold_seg_run<double, OpAdd, BLOCK>(d, r); break;
-      default: fold_seg_run<double, OpMax, BLOCK>(d, r); break;
+      case 0: fold_seg_run<int32_t, OpAdd, BLOCK>(d, r, s_seg); break;
+      case 1: fold_seg_run<int32_t, OpMax, BLOCK>(d, r, s_seg); break;
+      case 2: fold_seg_run<int64_t, OpAdd, BLOCK>(d, r, s_seg); break;
+      case 3: fold_seg_run<int64_t, OpMax, BLOCK>(d, r, s_seg); break;
+      case 4: fold_seg_run<float, OpAdd, BLOCK>(d, r, s_seg); break;
+      case 5: fold_seg_run<float, OpMax, BLOCK>(d, r, s_seg); break;
+      case 6: fold_seg_run<double, OpAdd, BLOCK>(d, r, s_seg); break;
+      default: fold_seg_run<double, OpMax, BLOCK>(d, r, s_seg); break;
     }
   } else {
     const bool i64 = d.aux64 != 0;
     switch (c >> 1) {
-      case 0: i64 ? hist_run<int32_t, int64_t, BLOCK>(d, st, r, epoch, (int32_t*)s_dyn)
-                  : hist_run<int32_t, int32_t, BLOCK>(d, st, r, epoch, (int32_t*)s_dyn); break;
-      case 1: i64 ? hist_run<int64_t, int64_t, BLOCK>(d, st, r, epoch, (int64_t*)s_dyn)
-                  : hist_run<int64_t, int32_t, BLOCK>(d, st, r, epoch, (int64_t*)s_dyn); break;
-      case 2: i64 ? hist_run<float, int64_t, BLOCK>(d, st, r, epoch, (float*)s_dyn)
-                  : hist_run<float, int32_t, BLOCK>(d, st, r, epoch, (float*)s_dyn); break;
-      default: i64 ? hist_run<double, int64_t, BLOCK>(d, st, r, epoch, (double*)s_dyn)
-                   : hist_run<double, int32_t, BLOCK>(d, st, r, epoch, (double*)s_dyn); break;
+      case 0: i64 ? hist_run<int32_t, int64_t, BLOCK>(d, st, r, epoch, s_dyn)
+                  : hist_run<int32_t, int32_t, BLOCK>(d, st, r, epoch, s_dyn); break;
+      case 1: i64 ? hist_run<int64_t, int64_t, BLOCK>(d, st, r, epoch, s_dyn)
+                  : hist_run<int64_t, int32_t, BLOCK>(d, st, r, epoch, s_dyn); break;
+      case 2: i64 ? hist_run<float, int64_t, BLOCK>(d, st, r, epoch, s_dyn)
+                  : hist_run<float, int32_t, BLOCK>(d, st, r, epoch, s_dyn); break;
+      default: i64 ? hist_run<double, int64_t, BLOCK>(d, st, r, epoch, s_dyn)
+                   : hist_run<double, int32_t, BLOCK>(d, st, r, epoch, s_dyn); break;
     }
   }
 }
@@ -319,14 +432,18 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
   extern __shared__ __align__(16) unsigned char s_dyn[];  // histogram bins (dynamic LDS)
   __shared__ uint64_t s_tick;
   __shared__ __align__(16) unsigned char s_raw[(BLOCK / 64 + 1) * sizeof(double)];
+  __shared__ SegTable s_seg;
   if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
   __syncthreads();
   const uint64_t tk0 = s_tick;
   const uint32_t epoch = (uint32_t)(tk0 >> 32);
   uint32_t g = (uint32_t)tk0;
   while (g < a.total_runs) {
+    // (the descriptors' pointers are loaded from memory, so every access
+    // through them names the global address space explicitly: flat accesses
+    // would also count against the LDS counter)
     const BatchDesc d = a.desc[a.run_prob[g]];
-    batch_run<BLOCK, K>(d, a.status + d.status_off, g - d.run0, epoch, s_raw, s_dyn);
+    batch_run<BLOCK, K>(d, a.status + d.status_off, g - d.run0, epoch, s_raw, s_dyn, &s_seg);
     __syncthreads();
     if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
     __syncthreads();
@@ -360,6 +477,8 @@ static int type_index(acc_type t) {
     default: return -1;
   }
 }
+
+bool batch_shape_supported(int block, int k) { return shape_kernel(block, k) != nullptr; }
 
 int batch_validate(const acc_batch_problem* problems, int64_t count) {
   for (int64_t i = 0; i < count; ++i) {
@@ -434,10 +553,15 @@ int batch_plan(const acc_batch_problem* problems, int64_t count, int block, int 
       d.ntiles = (uint32_t)p.m;
       nr = (uint64_t)((p.m + SEG_RUN - 1) / SEG_RUN);
     } else {
-      nr = p.n > 0 ? (uint64_t)((p.n + HIST_RUN - 1) / HIST_RUN) : 1;  // run 0 writes the initial bins
+      // runs over index positions relative to the 16-byte boundary below the
+      // indices; run 0 also writes the initial bins (so there is one even
+      // without indices)
+      const int64_t ib = p.aux_type == ACC_INT64 ? 8 : 4;
+      const int64_t shift = (int64_t)(((uintptr_t)p.in % 16) / ib);
+      nr = p.n > 0 ? (uint64_t)((shift + p.n + HIST_RUN - 1) / HIST_RUN) : 1;
       d.ntiles = (uint32_t)nr;
       words += 1;
-      lds = std::max(lds, (size_t)p.m * B);
+      lds = std::max(lds, (size_t)p.m * (p.type == ACC_FLOAT32 ? 8 : B));  // float32 bins accumulate in float64
     }
     d.run0 = (uint32_t)runs;
     runs += nr;
@@ -460,7 +584,7 @@ int batch_launch_plan(int device, const BatchPlan& plan, const void* d_desc, con
   const DeviceInfo* di;
   ACC_CHECK(device_info(device, &di));
   BatchArgs a{(const BatchDesc*)d_desc, (const uint32_t*)d_map, plan.total_runs, (SyncCtl*)d_ws,
-              (uint64_t*)((char*)d_ws + 256)};
+              (uint64_t*)((char*)d_ws + kSyncCtlBytes)};
   const void* kern = shape_kernel(plan.block, plan.k);
   if (!kern) return fail(ACC_ERROR_INVALID_VALUE, "batch: shape %d x %d", plan.block, plan.k);
   int per_cu = blocks_per_cu(kern, plan.block);
@@ -514,7 +638,7 @@ static int batch_layout(acc_batch_t b, int block, int k, int R) {
   const size_t n = b->problems.size();
   ACC_CHECK(dev_malloc(b->device, std::max<size_t>(n, 1) * sizeof(BatchDescBytes), &b->d_desc));
   ACC_CHECK(dev_malloc(b->device, std::max<size_t>(b->plan.run_prob.size(), 1) * sizeof(uint32_t), &b->d_map));
-  b->ws_bytes = 256 + b->plan.status_words * 8;
+  b->ws_bytes = kSyncCtlBytes + b->plan.status_words * 8;
   ACC_CHECK(dev_malloc(b->device, b->ws_bytes, &b->d_ws));
   ACC_HIP_CHECK(hipMemcpy(b->d_desc, b->plan.desc.data(), n * sizeof(BatchDescBytes), hipMemcpyHostToDevice));
   if (!b->plan.run_prob.empty())

@@ -18,11 +18,12 @@ struct BatchPlan {
   std::vector<uint32_t> run_prob;  // ticket -> problem
   int block = 256, k = 4, R = 4;
   uint32_t total_runs = 0;
-  uint64_t status_words = 0;       // 8-byte status words after the 256-byte SyncCtl
+  uint64_t status_words = 0;       // 8-byte status words after the SyncCtl (kSyncCtlBytes)
   size_t lds_bytes = 0;            // dynamic LDS (histogram bins)
 };
 
 int batch_validate(const acc_batch_problem* problems, int64_t count);
+bool batch_shape_supported(int block, int k);
 int batch_plan(const acc_batch_problem* problems, int64_t count, int block, int k, int R, BatchPlan* plan);
 int batch_launch_plan(int device, const BatchPlan& plan, const void* d_desc, const void* d_map, void* d_ws,
                       hipStream_t stream);

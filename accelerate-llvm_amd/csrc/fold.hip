@@ -121,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_fold_all(FoldAllArgs<T> a) {
     }
     *a.out = r.v;
     st_relaxed(&a.ctl->done, 0u);
-    __hip_atomic_store(&a.ctl->tick, (uint64_t)(epoch + 1u) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    advance_epoch(a.ctl, epoch);
   }
 }
 

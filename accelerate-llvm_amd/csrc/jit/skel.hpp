@@ -131,7 +131,7 @@ __device__ void fold_all(const Args& a, const Fn& fn, const Seed& seed, const Ld
     if constexpr (Seed::has) r = mc(fn, just(seed()), r);
     st(0, r.v);
     acc::st_relaxed(&ctl->done, 0u);
-    __hip_atomic_store(&ctl->tick, (uint64_t)(epoch + 1u) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc::advance_epoch(ctl, epoch);
   }
 }
 

@@ -11,6 +11,7 @@
 #include <memory>
 #include <atomic>
 #include "host.hpp"
+#include "lookback.hpp"
 #include "vecmem.hpp"
 #include "lookback.hpp"
 
@@ -140,7 +141,7 @@ int dev_free(int device, void* p) {
 }
 
 // ------------------------------------------------------------------ streams
-static constexpr size_t kCtlBytes = 256;
+static constexpr size_t kCtlBytes = kSyncCtlBytes;
 
 Workspace::~Workspace() {
   if (ptr) (void)dev_free(device, ptr);

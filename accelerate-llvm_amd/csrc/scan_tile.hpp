@@ -88,7 +88,7 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
       for (int j = 0; j < VEC; ++j) x[k][j] = r.e[j];
     } else {
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) x[k][j] = ok(k, j) ? a.in[e0 + j] : id;
+      for (int j = 0; j < VEC; ++j) x[k][j] = ok(k, j) ? gload(a.in + e0 + j) : id;
     }
   }
   // keep the loads above ahead of the arithmetic below
@@ -145,8 +145,8 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
     if (lane == 0) {
       if (t != a.last) Tagged<T>::publish(slot, incl, make_tag(epoch, ST_PREFIX));
       *s_prefix = X;
-      if (t == a.last && a.total_out) *a.total_out = incl;
-      if (t == a.first && a.seed_out) *a.seed_out = a.seed;
+      if (t == a.last && a.total_out) gstore(a.total_out, incl);
+      if (t == a.first && a.seed_out) gstore(a.seed_out, a.seed);
     }
   }
   __syncthreads();
@@ -170,7 +170,7 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
     } else if (vk) {
 #pragma unroll
       for (int j = 0; j < VEC; ++j)
-        if (vk >> j & 1u) a.out[e0 + j] = y[REVERSE ? VEC - 1 - j : j];
+        if (vk >> j & 1u) gstore(a.out + e0 + j, y[REVERSE ? VEC - 1 - j : j]);
     }
   }
 }

@@ -1,21 +1,27 @@
 // Serving loop: requests (one skeleton invocation each, inputs resident on
 // the device) arrive over time; one worker thread per execution slot admits
-// them as slots and device memory free up, runs admitted requests
-// concurrently on per-slot streams, and records each request's latency from
-// arrival to its result being complete on the device.
+// them as slots and device memory free up, runs admitted work concurrently on
+// per-slot streams, and records each request's latency from arrival to its
+// result being complete on the device.
 //
-// This is continuous batching in the sense of the serving literature: there
-// is no fixed batch boundary; a request that arrives while others execute is
-// admitted as soon as a slot and its output memory are available, and its
-// kernels overlap on the GPU with those already running.  Admission is by
-// free HBM: outputs are carved from a device arena sized at creation from
-// hipMemGetInfo (or the caller's budget, whichever is smaller); a request
-// whose output does not fit waits, and later small requests may pass it.
+// Continuous batching: there is no fixed batch boundary.  A slot that admits
+// a scan, fold, foldSeg or histogram also admits every other such request
+// that has arrived and whose output fits, up to a round limit, and runs the
+// whole round as ONE launch of the batched form's persistent kernel (one
+// descriptor upload + one launch per round instead of one or more launches
+// per request); requests that arrive while a round executes form the next
+// round on another slot, so rounds overlap on the GPU.  Stencils and the
+// combinations the batched kernel does not take run through the plain entry
+// points on their slot.  Admission is by free HBM: outputs are carved from a
+// device arena sized at creation from hipMemGetInfo (or the caller's budget,
+// whichever is smaller); a request whose output does not fit waits, and later
+// small requests may pass it.
 //
 // There is no counterpart in the reference (accelerate-llvm runs one program
-// per `run` call, PTX.hs:140-180); each request executes through the same
-// entry points as the plain calls, so "alone" and "in the mix" results come
-// from the same kernels.
+// per `run` call, PTX.hs:140-180).  A fused request runs the same per-problem
+// computation whatever else shares its round (its tiles, look-back order and
+// per-segment order depend only on the problem and the round's fixed shape),
+// so its result is the same alone and in the mix.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <atomic>
@@ -30,6 +36,8 @@
 #include <vector>
 #include <string.h>
 #include "host.hpp"
+#include "batch_internal.hpp"
+#include "lookback.hpp"
 
 namespace acc {
 
@@ -101,8 +109,25 @@ struct Req {
 
 using namespace acc;
 
+namespace acc {
+// Per-slot buffers of the fused rounds: pinned staging and device copies of
+// the descriptors and run map, and the batched kernel's workspace (SyncCtl +
+// status words).  Grown on demand, only while the slot has nothing in flight.
+struct RoundBufs {
+  void* h = nullptr;
+  size_t h_cap = 0;
+  void* d = nullptr;
+  size_t d_cap = 0;
+  void* ws = nullptr;
+  size_t ws_cap = 0;
+};
+}  // namespace acc
+
 struct acc_server_s {
   int device = 0;
+  acc_server_config cfg{1, 256, (int64_t)1 << 30, 256, 8, 4, 0};
+  std::vector<acc::RoundBufs> rounds_buf;  // one per slot
+  uint64_t rounds = 0, fused = 0;
   void* arena_base = nullptr;
   Arena arena;
   std::vector<acc_stream_t> slots;
@@ -176,36 +201,156 @@ static int run_request(acc_server_t s, Req& q, acc_stream_t st, void* out) {
   return fail(ACC_ERROR_UNSUPPORTED, "server: request kind %d", r.skeleton);
 }
 
+// The batched kernel's form of a request, if it has one: left-to-right
+// scans, folds and foldSegs with (+) or max, and (+) permutes of a constant
+// source into at most 4096 bins, over Int32/Int64/Float32/Float64, with the
+// alignment the batched kernel needs.
+static bool batchable(const acc_request& r, void* out, acc_batch_problem* p) {
+  if (r.type != ACC_INT32 && r.type != ACC_INT64 && r.type != ACC_FLOAT32 && r.type != ACC_FLOAT64) return false;
+  const bool add_max = r.op == ACC_OP_ADD || r.op == ACC_OP_MAX;
+  const bool idx_ok = (r.aux_type == ACC_INT32 || r.aux_type == ACC_INT64) && r.aux;
+  memset(p, 0, sizeof *p);
+  p->type = r.type;
+  p->op = r.op;
+  p->has_seed = r.has_seed ? 1 : 0;
+  p->seed = r.seed;
+  p->n = r.n;
+  p->out = out;
+  switch (r.skeleton) {
+    case ACC_REQ_SCAN:
+    case ACC_REQ_FOLD:
+      if (!add_max || r.n <= 0 || !r.in || (uintptr_t)r.in % 16 || (uintptr_t)out % 16) return false;
+      if (r.skeleton == ACC_REQ_SCAN && r.direction != ACC_LEFT_TO_RIGHT) return false;
+      p->kind = r.skeleton == ACC_REQ_SCAN ? ACC_BATCH_SCAN : ACC_BATCH_FOLD;
+      p->in = r.in;
+      return true;
+    case ACC_REQ_FOLD_SEG:
+      if (!add_max || r.m <= 0 || !idx_ok || (r.n > 0 && !r.in) || r.n < 0) return false;
+      p->kind = ACC_BATCH_FOLD_SEG;
+      p->in = r.in;
+      p->aux = r.aux;
+      p->aux_type = r.aux_type;
+      p->m = r.m;
+      return true;
+    case ACC_REQ_PERMUTE:
+      if (r.op != ACC_OP_ADD || r.in || r.m <= 0 || r.m > 4096 || !idx_ok || r.n < 0) return false;
+      p->kind = ACC_BATCH_HISTOGRAM;
+      p->in = r.aux;  // the indices
+      p->aux_type = r.aux_type;
+      p->m = r.m;
+      p->fill = r.fill;
+      p->seed = r.seed;  // every bin starts at the defaults value
+      return true;
+  }
+  return false;
+}
+
+static int grow_host(void** p, size_t* cap, size_t need) {
+  if (need <= *cap) return ACC_SUCCESS;
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  need = std::max<size_t>(need * 2, 64 << 10);
+  ACC_HIP_CHECK(hipHostMalloc(p, need, hipHostMallocDefault));
+  *cap = need;
+  return ACC_SUCCESS;
+}
+
+static int grow_device(int device, void** p, size_t* cap, size_t need) {
+  if (need <= *cap) return ACC_SUCCESS;
+  if (*p) dev_free(device, *p);
+  *p = nullptr;
+  *cap = 0;
+  need = std::max<size_t>(need * 2, 64 << 10);
+  ACC_CHECK(dev_malloc(device, need, p));
+  *cap = need;
+  return ACC_SUCCESS;
+}
+
+// One fused round on the slot's stream: plan, one upload of descriptors and
+// run map, one launch.  The slot's previous round has completed, so its
+// buffers can be reused (or replaced) freely.
+static int run_round(acc_server_t s, int slot, const std::vector<acc_batch_problem>& probs, BatchPlan& plan) {
+  const acc_server_config& c = s->cfg;
+  RoundBufs& b = s->rounds_buf[slot];
+  hipStream_t stream = s->slots[slot]->stream;
+  ACC_CHECK(batch_plan(probs.data(), (int64_t)probs.size(), c.block, c.vectors, c.run, &plan));
+  const size_t desc_bytes = (probs.size() * sizeof(BatchDescBytes) + 255) / 256 * 256;
+  const size_t map_bytes = plan.run_prob.size() * sizeof(uint32_t);
+  const size_t bytes = desc_bytes + map_bytes;
+  ACC_CHECK(grow_host(&b.h, &b.h_cap, bytes));
+  ACC_CHECK(grow_device(s->device, &b.d, &b.d_cap, bytes));
+  const size_t ws_bytes = kSyncCtlBytes + plan.status_words * 8;
+  if (ws_bytes > b.ws_cap) {
+    ACC_CHECK(grow_device(s->device, &b.ws, &b.ws_cap, ws_bytes));
+    // fresh memory: no stale word may carry the current epoch's tag
+    ACC_HIP_CHECK(hipMemsetAsync(b.ws, 0, b.ws_cap, stream));
+  }
+  memcpy(b.h, plan.desc.data(), probs.size() * sizeof(BatchDescBytes));
+  memcpy((char*)b.h + desc_bytes, plan.run_prob.data(), map_bytes);
+  ACC_HIP_CHECK(hipMemcpyAsync(b.d, b.h, bytes, hipMemcpyHostToDevice, stream));
+  return batch_launch_plan(s->device, plan, b.d, (char*)b.d + desc_bytes, b.ws, stream);
+}
+
 // One worker thread per slot: it takes the earliest arrived request whose
 // output fits in the free arena (a request that does not fit is passed by
-// later ones), launches it on the slot's stream, waits for its completion
-// event and retires it.  Workers launch in parallel, so the host-side cost of
-// the launches (a few microseconds each) is spread over the slots.
+// later ones).  If that request has a batched form (and fusion is on), the
+// slot also takes every other arrived request with a batched form whose
+// output fits, up to the round limits, and runs them as one fused round;
+// otherwise it runs the request through the plain entry points.  It then
+// waits for the slot's completion event and retires what it ran.  Workers
+// launch in parallel, so the host-side cost of the launches is spread over
+// the slots.
 static void slot_worker(acc_server_t s, int slot) {
   (void)hipSetDevice(s->device);
   acc_stream_t st = s->slots[slot];
   hipEvent_t ev = s->slot_event[slot];
+  std::vector<std::shared_ptr<Req>> round;
+  std::vector<acc_batch_problem> probs;
+  BatchPlan plan;
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     const auto now = clk::now();
-    std::shared_ptr<Req> pick;
-    size_t off = 0;
-    for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
-      const std::shared_ptr<Req>& q = s->reqs[*it];
+    round.clear();
+    probs.clear();
+    acc_batch_problem bp;
+    double round_bytes = 0;
+    for (size_t i = 0; i < s->pending.size();) {
+      const std::shared_ptr<Req>& q = s->reqs[s->pending[i]];
       if (q->arrival > now) break;  // pending is ordered by arrival
-      if (!s->arena.alloc(q->out_bytes, &off)) continue;
-      pick = q;
-      s->pending.erase(it);
-      break;
-    }
-    if (pick) {
-      Req& q = *pick;
-      q.out_off = off;
-      q.slot = slot;
-      s->slot_req[slot] = q.id;
+      const bool fusable = s->cfg.fuse != 0;
+      if (!round.empty()) {  // filling a fused round: batched forms only
+        if ((int)round.size() >= s->cfg.round_requests || round_bytes >= (double)s->cfg.round_bytes) break;
+        if (!fusable || !batchable(q->r, (void*)(uintptr_t)256, &bp)) {
+          ++i;
+          continue;
+        }
+      }
+      size_t off = 0;
+      if (!s->arena.alloc(q->out_bytes, &off)) {
+        ++i;
+        continue;
+      }
       void* out = (char*)s->arena_base + off;
+      q->out_off = off;
+      q->slot = slot;
+      const bool fused = fusable && batchable(q->r, out, &bp);
+      round.push_back(q);
+      s->pending.erase(s->pending.begin() + (std::ptrdiff_t)i);
+      if (!fused) break;  // a plain request runs alone
+      probs.push_back(bp);
+      round_bytes += request_bytes(q->r);
+    }
+    if (!round.empty()) {
+      s->slot_req[slot] = round.front()->id;
       lk.unlock();
-      int rc = run_request(s, q, st, out);
+      int rc;
+      if (probs.empty()) {
+        Req& q = *round.front();
+        rc = run_request(s, q, st, (char*)s->arena_base + q.out_off);
+      } else {
+        rc = run_round(s, slot, probs, plan);
+      }
       if (!rc && hipEventRecord(ev, st->stream) != hipSuccess) rc = ACC_ERROR_HIP;
       while (!rc) {  // poll: a blocking wait adds the scheduler's wake-up latency
         const hipError_t e = hipEventQuery(ev);
@@ -213,28 +358,37 @@ static void slot_worker(acc_server_t s, int slot) {
         if (e != hipErrorNotReady) rc = ACC_ERROR_HIP;
         else std::this_thread::yield();
       }
-      lk.lock();
-      q.done = clk::now();
-      q.finished = true;
-      s->slot_req[slot] = 0;
       if (rc) {
-        q.status = rc;
-        s->failed++;
-        // kernels queued before the failure may still write the range
-        lk.unlock();
+        // kernels queued before the failure may still write the outputs
         (void)hipStreamSynchronize(st->stream);
-        lk.lock();
-        s->arena.release(off, q.out_bytes);
-      } else {
+      }
+      const auto done = clk::now();
+      lk.lock();
+      s->slot_req[slot] = 0;
+      if (!probs.empty() && !rc) {
+        s->rounds++;
+        s->fused += probs.size();
+      }
+      for (auto& qp : round) {
+        Req& q = *qp;
+        q.done = done;
+        q.finished = true;
+        if (rc) {
+          q.status = rc;
+          s->failed++;
+          s->arena.release(q.out_off, q.out_bytes);
+          continue;
+        }
         s->completed++;
         s->latencies_us.push_back(std::chrono::duration<double, std::micro>(q.done - q.arrival).count());
         s->bytes_done += request_bytes(q.r);
         if (!s->any || q.done > s->last_done) s->last_done = q.done;
         if (!q.keep) {
           s->arena.release(q.out_off, q.out_bytes);
-          s->reqs.erase(q.id);  // q is gone from here on
+          s->reqs.erase(q.id);  // the round's reference (and a waiter's) keep q alive
         }
       }
+      round.clear();
       s->cv.notify_all();
       continue;
     }
@@ -267,6 +421,7 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
   cap = cap / 256 * 256;
   auto s = std::make_unique<acc_server_s>();
   s->device = device;
+  s->rounds_buf.resize(slots);
   if (cap) ACC_CHECK(dev_malloc(device, cap, &s->arena_base));
   s->arena.init(cap);
   for (int k = 0; k < slots; ++k) {
@@ -285,6 +440,25 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
   acc_server_s* raw = s.release();
   for (int k = 0; k < slots; ++k) raw->workers.emplace_back(slot_worker, raw, k);
   *out = raw;
+  return ACC_SUCCESS;
+}
+
+int acc_hip_server_configure(acc_server_t s, const acc_server_config* cfg) {
+  if (!s || !cfg) return fail(ACC_ERROR_INVALID_VALUE, "server: null argument");
+  acc_server_config c = *cfg;
+  if (c.fuse != 0 && c.fuse != 1) return fail(ACC_ERROR_INVALID_VALUE, "server: fuse must be 0 or 1");
+  if (!c.round_requests) c.round_requests = 256;
+  if (!c.round_bytes) c.round_bytes = (int64_t)1 << 30;
+  if (!c.block) c.block = 256;
+  if (!c.vectors) c.vectors = 8;
+  if (!c.run) c.run = 4;
+  if (c.round_requests < 1 || c.round_bytes < 1 || c.run < 1 || c.run > 64)
+    return fail(ACC_ERROR_INVALID_VALUE, "server: round limits and run length must be positive (run <= 64)");
+  if (!batch_shape_supported(c.block, c.vectors))
+    return fail(ACC_ERROR_INVALID_VALUE, "server: no batched kernel of %d threads x %d vectors", c.block, c.vectors);
+  c.reserved = 0;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->cfg = c;  // rounds formed from now on use it
   return ACC_SUCCESS;
 }
 
@@ -386,6 +560,8 @@ int acc_hip_server_stats(acc_server_t s, acc_server_stats* st) {
   }
   st->arena_bytes = s->arena.capacity();
   st->arena_used = s->arena.used();
+  st->rounds = s->rounds;
+  st->fused = s->fused;
   return ACC_SUCCESS;
 }
 
@@ -394,6 +570,7 @@ int acc_hip_server_reset_stats(acc_server_t s) {
   std::lock_guard<std::mutex> lk(s->mu);
   s->latencies_us.clear();
   s->completed = s->failed = 0;
+  s->rounds = s->fused = 0;
   s->bytes_done = 0;
   s->any = false;
   return ACC_SUCCESS;
@@ -410,6 +587,11 @@ int acc_hip_server_destroy(acc_server_t s) {
     if (w.joinable()) w.join();
   for (auto st : s->slots) acc_hip_stream_destroy(st);
   for (auto e : s->slot_event) (void)hipEventDestroy(e);
+  for (auto& b : s->rounds_buf) {
+    if (b.h) (void)hipHostFree(b.h);
+    if (b.d) dev_free(s->device, b.d);
+    if (b.ws) dev_free(s->device, b.ws);
+  }
   if (s->arena_base) dev_free(s->device, s->arena_base);
   delete s;
   return ACC_SUCCESS;

@@ -41,18 +41,21 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def timed(stream, fn, reps, warmup=2):
-    """Average milliseconds per call of fn() on `stream`, by HIP events."""
+def timed(stream, fn, reps=20, warmup=2, stat="median"):
+    """Median (or mean) milliseconds of `reps` calls of fn() on `stream`, each
+    bracketed by its own pair of HIP events on that stream (calls stay back to
+    back)."""
     for _ in range(warmup):
         fn()
-    e0, e1 = A.Event(stream.device), A.Event(stream.device)
+    ev = [A.Event(stream.device) for _ in range(reps + 1)]
     stream.synchronize()
-    e0.record(stream)
-    for _ in range(reps):
+    ev[0].record(stream)
+    for i in range(reps):
         fn()
-    e1.record(stream)
-    e1.synchronize()
-    return e0.elapsed_ms(e1) / reps
+        ev[i + 1].record(stream)
+    ev[-1].synchronize()
+    t = [ev[i].elapsed_ms(ev[i + 1]) for i in range(reps)]
+    return float(np.mean(t) if stat == "mean" else np.median(t))
 
 
 def seg_bytes(n, m, b=4, ib=8):
@@ -87,28 +90,33 @@ def bench_batched(s, args):
         n = int(sizes[i])
         probs.append((A.BATCH_FOLD, A.ADD, np.int32(0), d_xf.view(int(f_off[j]), n), n, d_yf.view(j * 64, 1)))
 
-    def per_call():
+    def per_call_python():
         for (kind, op, seed, src, n, out) in probs:
             if kind == A.BATCH_SCAN:
                 A.scan(s, op, A.LEFT_TO_RIGHT, None, src, 1, n, out)
             else:
                 A.fold(s, op, seed, src, 1, n, out)
 
-    per_call()
+    per_call_python()
     s.synchronize()
     e0, e1 = A.Event(0), A.Event(0)
     w0 = time.perf_counter()
     e0.record(s)
-    per_call()
+    per_call_python()
     e1.record(s)
     e1.synchronize()
-    percall_wall = time.perf_counter() - w0
-    percall_ms = e0.elapsed_ms(e1)
+    percall_py_wall = time.perf_counter() - w0
+    percall_py_ms = e0.elapsed_ms(e1)
     ref_fold = d_yf.to_numpy(s)[::64].copy()
     t0 = time.perf_counter()
     b = A.Batch(0, probs, tile=0)
     tune_s = time.perf_counter() - t0
     k, tiles = b.tile()
+    shape = b.shape()
+    # the per-call path from C: one plain entry-point call per problem, in
+    # order on the stream (no Python between the calls)
+    percall_ms = timed(s, lambda: b.launch_plain(s), reps=5, warmup=1)
+    assert np.array_equal(d_yf.to_numpy(s)[::64], ref_fold), "plain calls from C differ from the Python loop"
     g = A.Graph.capture(s, lambda: b.launch(s))
     g.launch(s)
     s.synchronize()
@@ -128,29 +136,24 @@ def bench_batched(s, args):
         assert np.all(np.abs(a - np.cumsum(x64)) <= bound), "batched scan outside the FP32 bound"
     want = np.sum(xf[int(f_off[0]):int(f_off[0]) + int(sizes[fold_idx[0]])], dtype=np.int32)
     assert got_fold[0] == want, "batched fold"
-    reps = 10
-    e0.record(s)
-    for _ in range(reps):
-        g.launch(s)
-    e1.record(s)
-    e1.synchronize()
-    batched_ms = e0.elapsed_ms(e1) / reps
+    batched_ms = timed(s, lambda: g.launch(s), reps=20)
     nbytes = sum(int(sizes[i]) * 8 for i in scan_idx) + sum(int(sizes[i]) * 4 + 4 for i in fold_idx)
     out = {
         "problems": P,
         "elements": int(sizes.sum()),
         "bytes": nbytes,
         "per_call_ms": round(percall_ms, 3),
-        "per_call_wall_ms": round(percall_wall * 1e3, 3),
+        "per_call_python_ms": round(percall_py_ms, 3),
+        "per_call_python_wall_ms": round(percall_py_wall * 1e3, 3),
         "batched_ms": round(batched_ms, 3),
         "speedup": round(percall_ms / batched_ms, 2),
         "batched_gbps": round(nbytes / (batched_ms * 1e-3) / 1e9, 1),
-        "tile_vectors_per_thread": k,
-        "tiles": tiles,
+        "shape": {"block": shape[0], "vectors_per_thread": shape[1], "tiles_per_run": shape[2],
+                  "tickets": shape[3]},
         "autotune_s": round(tune_s, 2),
     }
-    log(f"batched: per-call {percall_ms:.2f} ms (wall {percall_wall*1e3:.1f} ms), batched {batched_ms:.3f} ms "
-        f"-> {out['speedup']}x, {out['batched_gbps']} GB/s (tile K={k})")
+    log(f"batched: per-call from C {percall_ms:.2f} ms (Python loop {percall_py_ms:.2f} ms), batched "
+        f"{batched_ms:.3f} ms -> {out['speedup']}x, {out['batched_gbps']} GB/s (shape {shape})")
     g.destroy()
     b.destroy()
     return out
@@ -200,22 +203,42 @@ def bench_serving(s, args):
                                          boundary=A.MIRROR, weights=w))
             nbytes += 2 * side * side * B
 
-    # the plain per-call path: each request runs alone, the client waits for it
+    # the plain per-call path, from C: one slot, no fusion -- each request runs
+    # alone through the plain entry points and completes before the next
+    # starts (the client waits for each)
+    def capacity_of(srv):
+        # one arrival instant for all, far enough ahead that the submit loop
+        # (a few microseconds per request from Python) has finished by then
+        lead = max(50_000_000, 40_000 * len(reqs))
+        t_arrive = time.perf_counter_ns() + lead
+        for q in reqs:
+            srv.submit(q, arrival_offset_ns=t_arrive - time.perf_counter_ns())
+        assert time.perf_counter_ns() < t_arrive, "submit loop slower than the arrival lead time"
+        srv.drain()
+        st = srv.stats()
+        srv.reset_stats()
+        return st
+
+    plain_srv = A.Server(0, slots=1, fuse=False)
+    plain = capacity_of(plain_srv).req_per_s
+    plain_srv.destroy()
+    # and from Python (ctypes call + synchronize per request), for reference
     plain_t0 = time.perf_counter()
     outs = A.DeviceArray(N * 8, np.uint8)
     for (r, w) in reqs:
         run_plain(s, r, w, outs)
         s.synchronize()
-    plain_s = time.perf_counter() - plain_t0
+    plain_py = R / (time.perf_counter() - plain_t0)
 
-    srv = A.Server(0, slots=args.serve_slots)
+    unfused_srv = A.Server(0, slots=args.serve_slots, fuse=False)
+    unfused = capacity_of(unfused_srv).req_per_s
+    unfused_srv.destroy()
+
+    srv = A.Server(0, slots=args.serve_slots, round_requests=args.serve_round_requests,
+                   round_bytes=args.serve_round_mib << 20, shape=(256, args.serve_vectors, 4))
     # capacity: every request arrives at once
-    for q in reqs:
-        srv.submit(q)
-    srv.drain()
-    cap = srv.stats()
+    cap = capacity_of(srv)
     capacity = cap.req_per_s
-    srv.reset_stats()
     # open-loop Poisson trace at 80% of capacity: arrival times are measured
     # from one origin (each submit gives its offset from the moment it is made)
     lam = 0.8 * capacity
@@ -236,11 +259,17 @@ def bench_serving(s, args):
         "p99_ms": round(st.p99_us / 1e3, 3),
         "mean_ms": round(st.mean_us / 1e3, 3),
         "gbps": round(st.bytes_per_s / 1e9, 1),
-        "plain_req_per_s": round(R / plain_s, 1),
-        "capacity_vs_plain": round(capacity / (R / plain_s), 2),
+        "fused_rounds_at_capacity": int(cap.rounds),
+        "fused_requests_at_capacity": int(cap.fused),
+        "plain_req_per_s": round(plain, 1),
+        "plain_python_req_per_s": round(plain_py, 1),
+        "unfused_slots_req_per_s": round(unfused, 1),
+        "capacity_vs_plain": round(capacity / plain, 2),
     }
-    log(f"serving: capacity {capacity:.0f} req/s (plain per-call {R/plain_s:.0f} req/s); at {lam:.0f} req/s "
-        f"p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, {out['req_per_s']} req/s")
+    log(f"serving: capacity {capacity:.0f} req/s in {cap.rounds} fused rounds ({cap.fused} fused requests); "
+        f"plain per-call from C {plain:.0f} req/s (Python {plain_py:.0f}), unfused {args.serve_slots} slots "
+        f"{unfused:.0f} req/s; at {lam:.0f} req/s p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, "
+        f"{out['req_per_s']} req/s")
     return out
 
 
@@ -343,7 +372,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--skeleton-reps", type=int, default=10)
+    ap.add_argument("--skeleton-reps", type=int, default=20, help="timed calls per skeleton (median reported)")
     ap.add_argument("--no-skeletons", action="store_true", help="skip the per-skeleton lines")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -352,6 +381,9 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=10000)
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=8, help="server execution slots (one worker thread each)")
+    ap.add_argument("--serve-round-requests", type=int, default=256, help="requests per fused round at most")
+    ap.add_argument("--serve-round-mib", type=int, default=1024, help="algorithmic MiB per fused round at most")
+    ap.add_argument("--serve-vectors", type=int, default=8, help="vectors per thread of the fused launch (2, 4, 8)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--kernel-only", action="store_true",
                     help="(internal) run only the dominant kernel and a 1 GiB copy, for counter collection")
@@ -406,7 +438,19 @@ def main():
     bound = 1e-5 * np.maximum(1.0, np.log2(np.maximum(np.diff(off), 2))) * ab
     err = np.abs(d_sum.to_numpy(s).astype(np.float64) - truth)
     assert np.all(err <= bound), "fold1Seg (+) outside the FP32 bound"
-    log(f"step outputs checked (fold1Seg (+) max error/bound {float((err / bound).max()):.3f})")
+    rel = err / np.maximum(ab, 1e-300)
+    fp_error = {
+        "result": "fold1Seg (+) Float32, one value per segment, against the FP64 sum",
+        "results": int(N_SEG),
+        "max_rel": float(rel.max()),
+        "p99_rel": float(np.quantile(rel, 0.99)),
+        "median_rel": float(np.median(rel)),
+        "rel": "|y - s| / sum|x| per segment",
+        "bound": "1e-5 * max(1, log2(len)) * sum|x|",
+        "max_err_over_bound": float((err / bound).max()),
+    }
+    log(f"step outputs checked (fold1Seg (+) per segment |y-s|/sum|x|: max {fp_error['max_rel']:.3e} "
+        f"p99 {fp_error['p99_rel']:.3e}; max error/bound {fp_error['max_err_over_bound']:.3f})")
 
     # ------------------------------------------------------------ timed steps
     e0, e1 = A.Event(0), A.Event(0)
@@ -424,7 +468,9 @@ def main():
 
     # ---------------------------------------------- dominant kernel roofline
     k_bytes = seg_bytes(N_VALUES, N_SEG)
-    k_ms = timed(s, lambda: A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum), 50)
+    k_fn = lambda: A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)
+    k_ms = timed(s, k_fn, 50, stat="mean")
+    k_med = timed(s, k_fn, 20)
     k_gbs = k_bytes / (k_ms * 1e-3) / 1e9
     roofline = {
         "kernel": "fold1Seg (+) Float32 call: k_tile_seg + k_fold_seg + k_seg_fix",
@@ -436,6 +482,7 @@ def main():
         "traffic": None,
         "algorithmic_bytes": k_bytes,
         "avg_us": round(k_ms * 1e3, 2),
+        "median_us": round(k_med * 1e3, 2),
     }
     log(f"fold1Seg (+) kernel {k_ms * 1e3:.1f} us -> {k_gbs:.0f} GB/s")
     if traffic:
@@ -578,6 +625,7 @@ def main():
         },
         "frac_of_hbm_peak": round(value / HBM_PEAK_GBS, 4),
         "roofline": roofline,
+        "fp_error": fp_error,
         "cpu_baseline": cpu,
         "skeletons": skel,
         "batched": batched,

@@ -606,6 +606,8 @@ typedef struct acc_server_stats {
   double p50_us, p99_us, mean_us, max_us;
   double req_per_s, bytes_per_s, span_s; /* span: first arrival to last completion */
   size_t arena_bytes, arena_used;
+  uint64_t rounds; /* fused rounds run (one batched launch each) */
+  uint64_t fused;  /* requests that ran in a fused round */
 } acc_server_stats;
 
 typedef struct acc_server_s* acc_server_t;
@@ -618,6 +620,27 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
  * request it returns the output's device address; for one submitted with
  * keep_output == 0 it returns *out = NULL, and once such a request has
  * completed and been retired its id is unknown (ACC_ERROR_INVALID_VALUE). */
+/* Fused rounds.  With fuse == 1 (the default) a slot that admits a request
+ * with a batched form -- a left-to-right scan, a fold or a foldSeg with (+) or
+ * max, or a (+) permute of a constant source (in == NULL) into at most 4096
+ * bins, over INT32/INT64/FLOAT32/FLOAT64, scan and fold inputs 16-byte
+ * aligned -- also admits every other arrived request with a batched form whose
+ * output fits, up to round_requests requests and round_bytes algorithmic
+ * bytes, and runs them as ONE launch of the batched kernel (acc_hip_batch_*)
+ * with the given shape; other requests run through the plain entry points.  A
+ * fused request's result depends only on the request and the shape, not on
+ * what else shares its round.  fuse == 0 runs every request through the plain
+ * entry points, one per slot at a time (the per-call path).  Zero fields take
+ * the defaults: 256 requests, 1 GiB, 256 threads x 8 vectors, runs of 4 tiles.
+ * Applies to rounds formed after the call. */
+typedef struct acc_server_config {
+  int32_t fuse;
+  int32_t round_requests;
+  int64_t round_bytes;
+  int32_t block, vectors, run; /* the batched kernel's shape (acc_hip_batch_shape) */
+  int32_t reserved;
+} acc_server_config;
+int acc_hip_server_configure(acc_server_t s, const acc_server_config* cfg);
 int acc_hip_server_submit(acc_server_t s, const acc_request* r, int64_t arrival_offset_ns, int keep_output,
                           uint64_t* id);
 int acc_hip_server_wait(acc_server_t s, uint64_t id, void** out, double* latency_us);

@@ -1,5 +1,7 @@
 """Serving loop: requests admitted as slots and memory free up, overlapping
 on the device; each result equals the same request run alone."""
+import time
+
 import numpy as np
 import pytest
 
@@ -59,6 +61,11 @@ def expected(pool, kind, dt, key, size):
     x = pool["host"][key]
     if kind == "scan":
         return O.scan(A.ADD, 0, None, x[:size])
+    if kind == "scanr_max":
+        return O.scan(A.MAX, 1, None, x[:size])
+    if kind == "fold1seg_min":
+        ne, nseg = size
+        return O.fold_seg(A.MIN, None, x[:ne], pool["off_host"][:nseg + 1])
     if kind == "fold1seg":
         ne, nseg = size
         return O.fold_seg(A.ADD, None, x[:ne], pool["off_host"][:nseg + 1])
@@ -70,7 +77,7 @@ def expected(pool, kind, dt, key, size):
 
 
 def agree(got, want, kind, dt, pool, key, size):
-    if np.dtype(dt).kind != "f" or kind == "hist":
+    if np.dtype(dt).kind != "f" or kind in ("hist", "scanr_max", "fold1seg_min"):
         return np.array_equal(got, want)
     x = pool["host"][key]
     g = got.astype(np.float64)
@@ -155,4 +162,85 @@ def test_server_wait_on_a_request_that_is_not_kept(stream, pool):
         assert ptr is None and lat > 0
     srv.drain()
     assert srv.stats().completed == 8
+    srv.destroy()
+
+
+def out_count(kind, size):
+    if kind in ("fold1seg", "fold1seg_min"):
+        return size[1]
+    if kind == "hist":
+        return 4096
+    if kind == "stencil":
+        return size * size
+    return size
+
+
+def test_fused_rounds_same_alone_and_in_the_mix(stream, pool):
+    """All requests arrive at once, so slots take them in fused rounds (one
+    batched launch per round); requests without a batched form (a right-to-
+    left scan, a min fold1Seg, the stencils) run through the plain calls
+    beside them.  Every result agrees with the oracle, and each request run
+    alone (a round of its own) gives the same result: bit-exact for integers
+    and histograms, within the FP32 bound for floating (+) (the look-back
+    may associate the tile sums differently from run to run)."""
+    reqs = mix(pool, 90, seed=2)
+    off = pool["off_host"]
+    for k, key in enumerate(("f32", "i32", "i64")):
+        dt = {"f32": np.float32, "i32": np.int32, "i64": np.int64}[key]
+        n = 5000 + 777 * k
+        reqs.append(("scanr_max", dt, key, n,
+                     A.Server.request(A.REQ_SCAN, dt, A.MAX, src=pool[key], n=n, direction=1)))
+        nseg = 300 + k
+        reqs.append(("fold1seg_min", dt, key, (int(off[nseg]), nseg),
+                     A.Server.request(A.REQ_FOLD_SEG, dt, A.MIN, src=pool[key], n=int(off[nseg]), m=nseg,
+                                      aux=pool["off"])))
+    order = np.random.default_rng(3).permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+    srv = A.Server(0, slots=2)
+    # one arrival instant for all (30 ms ahead), so the rounds are formed
+    # from the whole set rather than from whatever the submit loop has reached
+    t_arrive = time.perf_counter_ns() + 30_000_000
+    ids = [srv.submit(r, keep=True, arrival_offset_ns=t_arrive - time.perf_counter_ns()) for (*_, r) in reqs]
+    mixed = []
+    for rid, (kind, dt, key, size, r) in zip(ids, reqs):
+        ptr, _ = srv.wait(rid)
+        got = A.copy_to_host(stream, ptr, out_count(kind, size), dt)
+        assert agree(got, expected(pool, kind, dt, key, size), kind, dt, pool, key, size), (kind, dt, size)
+        mixed.append(got)
+        srv.release(rid)
+    st = srv.stats()
+    fusable = sum(k in ("scan", "fold1seg", "hist") for (k, *_) in reqs)
+    assert st.completed == len(reqs) and st.failed == 0
+    assert st.fused == fusable and 1 <= st.rounds < fusable, (st.rounds, st.fused, fusable)
+    srv.destroy()
+
+    alone = A.Server(0, slots=1)
+    for (kind, dt, key, size, r), m in zip(reqs, mixed):
+        rid = alone.submit(r, keep=True)
+        ptr, _ = alone.wait(rid)
+        got = A.copy_to_host(stream, ptr, out_count(kind, size), dt)
+        alone.release(rid)
+        if np.dtype(dt).kind != "f" or kind not in ("scan", "fold1seg"):
+            assert np.array_equal(got, m), (kind, np.dtype(dt).name, size)
+        else:
+            assert agree(got, expected(pool, kind, dt, key, size), kind, dt, pool, key, size)
+    st = alone.stats()
+    assert st.fused == fusable and st.rounds == fusable
+    alone.destroy()
+
+
+def test_unfused_server_is_the_plain_path(stream, pool):
+    """fuse=False: every request through the plain entry points."""
+    reqs = mix(pool, 30, seed=4)
+    srv = A.Server(0, slots=2, fuse=False)
+    ids = [srv.submit(r, keep=True) for (*_, r) in reqs]
+    for rid, (kind, dt, key, size, r) in zip(ids, reqs):
+        ptr, _ = srv.wait(rid)
+        got = A.copy_to_host(stream, ptr, out_count(kind, size), dt)
+        assert agree(got, expected(pool, kind, dt, key, size), kind, dt, pool, key, size)
+        srv.release(rid)
+    st = srv.stats()
+    assert st.completed == 30 and st.rounds == 0 and st.fused == 0
+    with pytest.raises(A.AccError):
+        srv.configure(shape=(128, 8, 4))  # no such batched kernel
     srv.destroy()

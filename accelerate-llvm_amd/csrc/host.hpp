@@ -45,7 +45,7 @@ int dev_free(int device, void* p);
 }  // namespace acc
 
 namespace acc {
-// One device allocation of skeleton workspace: [SyncCtl (4 KiB) | status
+// One device allocation of skeleton workspace: [SyncCtl (256 B) | status
 // words].  Shared ownership: the stream that allocated it and every graph
 // captured on that stream while it was current keep it alive, since a graph's
 // kernels hold its address as frozen arguments.
@@ -99,7 +99,7 @@ namespace acc {
 int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status);
 
 // Routes the skeleton workspace of the calling thread's launches to a
-// caller-provided device region [SyncCtl (4 KiB) | status words] for the
+// caller-provided device region [SyncCtl (256 B) | status words] for the
 // lifetime of the object (the allocation-free entry points).
 struct ScratchOverride {
   ScratchOverride(int device, void* base, size_t bytes);

@@ -21,16 +21,13 @@
 // finished workgroups: no forward-progress assumption about workgroups that
 // have not been dispatched.
 //
-// The tickets are striped over S = min(kTicketStripes, grid) counters, one per
-// 128-byte line: one counter hands out about one ticket per 12 ns (measured;
-// atomics to one line serialise), sixteen about one per 0.9 ns.  Workgroup b
-// takes from stripe s = b % S, whose k-th ticket is k * S + s.  Each stripe
-// hands out its tickets in increasing order, and workgroups 0 .. S-1 (one per
-// stripe) are dispatched first, so every stripe that still has tickets keeps
-// a running workgroup; the lowest unfinished ticket is then always either
-// held by a running workgroup whose predecessors are all finished, or the
-// next ticket of a stripe whose workgroups hold no unfinished ticket, and the
-// launch cannot deadlock.
+// One counter hands out the tickets (about one per 12 ns, measured; atomics
+// to one 128-byte line serialise).  Striping them over several counters
+// would multiply that, but a stripe's next ticket can then only be taken by
+// one of the stripe's own workgroups: two persistent launches that are each
+// partly resident (concurrent streams) could then hold the slots each
+// other's missing stripe needs.  With one counter every dispatched workgroup
+// takes the next ticket, so the running ones always hold a prefix.
 #pragma once
 #ifndef __HIPCC_RTC__  // hiprtc provides the HIP runtime itself
 #include <hip/hip_runtime.h>
@@ -40,22 +37,15 @@
 
 namespace acc {
 
-constexpr int kTicketStripes = 16;
-
-struct alignas(128) TicketLine {
-  uint64_t tick;  // high 32 bits: launch epoch; low 32 bits: the stripe's next ticket number
-  uint64_t pad[15];
-};
-
-struct alignas(128) SyncCtl {
-  TicketLine line[kTicketStripes];
+struct alignas(64) SyncCtl {
+  uint64_t tick;  // high 32 bits: launch epoch; low 32 bits: next tile ticket
   uint32_t done;  // workgroups finished in this launch
-  uint32_t pad[31];
+  uint32_t pad[13];
 };
 
 // bytes reserved for the SyncCtl at the start of a workspace (the status
 // words follow)
-constexpr size_t kSyncCtlBytes = 4096;
+constexpr size_t kSyncCtlBytes = 256;
 static_assert(sizeof(SyncCtl) <= kSyncCtlBytes, "SyncCtl size");
 
 enum : uint32_t { ST_INVALID = 0, ST_AGGREGATE = 1, ST_PREFIX = 2 };
@@ -102,30 +92,24 @@ template <typename T> struct Tagged {
   }
 };
 
-// Take a ticket (thread 0): high 32 bits the epoch, low 32 bits the ticket.
-// One atomic gives both, so a tile costs one round trip here.
+// Take a ticket (thread 0) and broadcast it: high 32 bits epoch, low 32 bits
+// the ticket.  One atomic gives both, so a tile costs one round trip here.
 __device__ __forceinline__ uint64_t take_ticket(SyncCtl* ctl) {
-  const uint32_t S = gridDim.x < (uint32_t)kTicketStripes ? gridDim.x : (uint32_t)kTicketStripes;
-  const uint32_t s = blockIdx.x % S;
-  const uint64_t t =
-      __hip_atomic_fetch_add(&ctl->line[s].tick, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return (t & 0xffffffff00000000ull) | (uint32_t)((uint32_t)t * S + s);
+  return __hip_atomic_fetch_add(&ctl->tick, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t block_epoch(SyncCtl* ctl, uint32_t* lds_slot) {
   if (threadIdx.x == 0)
-    *lds_slot = (uint32_t)(__hip_atomic_load(&ctl->line[0].tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
+    *lds_slot = (uint32_t)(__hip_atomic_load(&ctl->tick, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
   __syncthreads();
   uint32_t e = *lds_slot;
   __syncthreads();
   return e;
 }
 
-// The next launch's epoch, with every stripe's ticket back at 0 (all
-// stripes, whatever this launch's grid used).
+// The next launch's epoch, with the ticket back at 0.
 __device__ __forceinline__ void advance_epoch(SyncCtl* ctl, uint32_t epoch) {
-  for (int s = 0; s < kTicketStripes; ++s)
-    __hip_atomic_store(&ctl->line[s].tick, (uint64_t)(epoch + 1u) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&ctl->tick, (uint64_t)(epoch + 1u) << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Called once per workgroup after its last use of the status words.  The last

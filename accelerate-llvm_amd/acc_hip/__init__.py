@@ -90,12 +90,12 @@ class BatchProblem(ctypes.Structure):
         ("aux", c_void_p),
         ("m", ctypes.c_int64),
         ("aux_type", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("params", ctypes.c_int32),
         ("fill", ctypes.c_uint64),
     ]
 
 
-BATCH_SCAN, BATCH_FOLD, BATCH_FOLD_SEG, BATCH_HISTOGRAM = 0, 1, 2, 3
+BATCH_SCAN, BATCH_FOLD, BATCH_FOLD_SEG, BATCH_HISTOGRAM, BATCH_STENCIL = 0, 1, 2, 3, 4
 
 REQ_FOLD, REQ_SCAN, REQ_FOLD_SEG, REQ_PERMUTE, REQ_STENCIL2D = range(5)
 
@@ -558,13 +558,15 @@ class Batch:
     """Many scans/folds in one persistent launch (acc_hip_batch_*)."""
 
     def __init__(self, device, problems, tile=0):
-        """problems: list of (kind, op, seed_or_None, src, n, out[, aux, m, fill]): src/out
-        DeviceArray/View; foldSeg: aux = offsets, m = segments; histogram: src = indices,
-        out = m bins of the result type, seed = their initial value, fill = the source value."""
+        """problems: list of (kind, op, seed_or_None, src, n, out[, aux, m, fill, params]):
+        src/out DeviceArray/View; foldSeg: aux = offsets, m = segments; histogram: src =
+        indices, out = m bins of the result type, seed = their initial value, fill = the
+        source value; stencil: an n x m image, aux = the weights on the device, fill = the
+        constant boundary's value, params = ry | rx << 8 | boundary << 16."""
         arr = (BatchProblem * len(problems))()
         for i, prob in enumerate(problems):
             kind, op, seed, src, n, out = prob[:6]
-            aux, m, fill = (list(prob[6:]) + [None, 0, 0])[:3]
+            aux, m, fill, params = list(prob[6:10]) + [None, 0, 0, 0][len(prob[6:10]):]
             dt = out.dtype if kind == BATCH_HISTOGRAM else src.dtype
             arr[i].kind = kind
             arr[i].type = acc_type(dt)
@@ -580,7 +582,8 @@ class Batch:
             elif kind == BATCH_HISTOGRAM:
                 arr[i].aux_type = acc_type(src.dtype)
             arr[i].m = m
-            arr[i].fill = _bits(dt, fill) if kind == BATCH_HISTOGRAM else 0
+            arr[i].fill = _bits(dt, fill) if kind in (BATCH_HISTOGRAM, BATCH_STENCIL) else 0
+            arr[i].params = params
         h = c_void_p()
         check(lib().acc_hip_batch_create(device, arr, len(problems), tile, ctypes.byref(h)))
         self.handle = h

@@ -44,11 +44,12 @@
 #include <algorithm>
 #include <chrono>
 #include "scan_tile.hpp"
+#include "boundary.hpp"
 #include "batch_internal.hpp"
 
 namespace acc {
 
-enum : int32_t { BK_SCAN = 0, BK_FOLD = 1, BK_FOLD_SEG = 2, BK_HIST = 3 };
+enum : int32_t { BK_SCAN = 0, BK_FOLD = 1, BK_FOLD_SEG = 2, BK_HIST = 3, BK_STENCIL = 4 };
 
 struct BatchDesc {  // one device descriptor per problem (96 bytes, asserted below)
   int32_t cls;          // kind * 8 + type index * 2 + op index
@@ -58,13 +59,14 @@ struct BatchDesc {  // one device descriptor per problem (96 bytes, asserted bel
   void* out;
   uint64_t seed;        // bit pattern
   uint64_t status_off;  // first status word of this problem
-  uint32_t ntiles;      // scan/fold: tiles; foldSeg: segments; histogram: indices / HIST_RUN
+  uint32_t ntiles;      // scan/fold: tiles; foldSeg: segments; histogram: indices / HIST_RUN;
+                        // stencil: column strips
   uint32_t run0;        // first ticket of this problem
-  uint32_t R;           // scan/fold: tiles per run
+  uint32_t R;           // scan/fold: tiles per run; stencil: boundary
   uint32_t aux64;       // foldSeg offsets / histogram indices are Int64
-  const void* aux;      // foldSeg: offsets
-  int64_t m;            // foldSeg: segments; histogram: bins
-  uint64_t fill;        // histogram: constant source value
+  const void* aux;      // foldSeg: offsets; stencil: weights
+  int64_t m;            // foldSeg: segments; histogram: bins; stencil: columns (n: rows)
+  uint64_t fill;        // histogram: constant source value; stencil: the constant boundary's value
   uint64_t reserved;    // pads to the host's descriptor stride (96 bytes)
 };
 
@@ -385,6 +387,135 @@ __device__ __forceinline__ void hist_run(const BatchDesc& d, uint64_t* status, u
   __syncthreads();
 }
 
+// Stencil (weighted sum over a (2RY+1) x (2RX+1) neighbourhood, taps in
+// tuple order: rows outer, columns inner, LLVM/CodeGen/Stencil.hs:126-180;
+// boundaries as in LLVM/CodeGen/Stencil.hs:224-253).  A run is a strip of
+// ST_STRIP columns by BLOCK/64 * ST_ROWS rows; each wave slides down ST_ROWS
+// output rows.  Lane l holds columns l + 64 j (j < ST_NJ) of the strip, so
+// every row is read with ST_NJ fully coalesced loads per lane and each input
+// row once per wave; horizontal neighbours come from the neighbouring lanes
+// (ds_bpermute), the two halo columns by a broadcast load.  Each input row is
+// added into the 2RY+1 output rows that use it (accumulators rotate in
+// registers), which keeps every output's taps in tuple order.  Row and column
+// indices outside the image go through the boundary map (the constant
+// boundary substitutes its value), so no element is special-cased.
+constexpr int ST_NJ = 2;             // columns per lane
+constexpr int ST_STRIP = 64 * ST_NJ;  // columns per run
+constexpr int ST_ROWS = 16;    // output rows per wave per run
+
+template <typename T> __device__ __forceinline__ T lane_fetch(const T& v, int src) {
+  Words<T> a = to_words(v), r;
+#pragma unroll
+  for (int i = 0; i < Words<T>::N; ++i) r.w[i] = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)a.w[i]);
+  return from_words<T>(r);
+}
+
+template <typename T, int RY, int RX, int BLOCK>
+__device__ __forceinline__ void stencil_run(const BatchDesc& d, uint32_t r) {
+  constexpr int KH = 2 * RY + 1, KW = 2 * RX + 1;
+  OpAdd add{};
+  OpMul mul{};
+  const int lane = lane_id(), w = threadIdx.x / 64;
+  const int64_t H = d.n, Wd = d.m;
+  const int64_t strip = r % d.ntiles, rb = r / d.ntiles;
+  const int64_t c0 = strip * ST_STRIP;
+  const int64_t y0 = (rb * (BLOCK / 64) + w) * ST_ROWS, y1 = min(H, y0 + ST_ROWS);
+  if (y0 >= y1) return;  // wave-uniform; no barrier below
+  const int bnd = (int)d.R;
+  const bool cst = bnd == ACC_BOUNDARY_CONSTANT;
+  const T cval = bits_to<T>(d.fill);
+  const T* in = (const T*)d.in;
+  T* out = (T*)d.out;
+  T wt[KH * KW];
+#pragma unroll
+  for (int i = 0; i < KH * KW; ++i) wt[i] = gload((const T*)d.aux + i);
+  // the lane's columns (mapped) and whether each is an output column
+  // (column indices fit 32 bits: the batch takes images of < 2^30 columns)
+  int32_t col[ST_NJ];
+  bool live[ST_NJ];
+#pragma unroll
+  for (int j = 0; j < ST_NJ; ++j) {
+    const int64_t c = c0 + lane + 64 * j;
+    live[j] = c < Wd;
+    col[j] = (int32_t)(live[j] ? c : bound_index(bnd, c, Wd));
+  }
+  // halo columns: c0 - RX .. c0 - 1 and c0 + ST_STRIP .. + RX - 1
+  int32_t hcol[2 * RX];
+  bool hout[2 * RX];
+#pragma unroll
+  for (int k = 0; k < 2 * RX; ++k) {
+    const int64_t c = k < RX ? c0 - RX + k : c0 + ST_STRIP + (k - RX);
+    hout[k] = c < 0 || c >= Wd;
+    hcol[k] = (int32_t)bound_index(bnd, c, Wd);
+  }
+  T acc[KH][ST_NJ];  // acc[k]: output row yi - RY + k while input row yi is added
+#pragma unroll
+  for (int k = 0; k < KH; ++k)
+#pragma unroll
+    for (int j = 0; j < ST_NJ; ++j) acc[k][j] = T();
+  for (int64_t yi = y0 - RY; yi < y1 + RY; ++yi) {
+    const bool rout = yi < 0 || yi >= H;
+    const T* row = in + bound_index(bnd, yi, H) * Wd;
+    T x[ST_NJ], h[2 * RX];
+#pragma unroll
+    for (int j = 0; j < ST_NJ; ++j) x[j] = (cst && (rout || !live[j])) ? cval : gload(row + col[j]);
+#pragma unroll
+    for (int k = 0; k < 2 * RX; ++k) h[k] = (cst && (rout || hout[k])) ? cval : gload(row + hcol[k]);
+#pragma unroll
+    for (int dx = -RX; dx <= RX; ++dx) {
+      // e[j]: the value at column (lane + 64 j) + dx of the strip
+      T e[ST_NJ];
+      if (dx == 0) {
+#pragma unroll
+        for (int j = 0; j < ST_NJ; ++j) e[j] = x[j];
+      } else {
+        const int src = (lane + dx) & 63;
+        T sh[ST_NJ];
+#pragma unroll
+        for (int j = 0; j < ST_NJ; ++j) sh[j] = lane_fetch(x[j], src);
+        const bool wrap = dx < 0 ? lane + dx < 0 : lane + dx > 63;
+        // the halo value of a wrapping lane at the strip's edge
+        T he = h[0];
+#pragma unroll
+        for (int k = 0; k < 2 * RX; ++k) {
+          const int hk = dx < 0 ? RX + lane + dx : RX + lane + dx - 64;  // index into h
+          if (hk == k) he = h[k];
+        }
+#pragma unroll
+        for (int j = 0; j < ST_NJ; ++j) {
+          const T nb = dx < 0 ? (j > 0 ? sh[j - 1] : he) : (j < ST_NJ - 1 ? sh[j + 1] : he);
+          e[j] = wrap ? nb : sh[j];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KH; ++k) {
+        const int dy = RY - k;  // this input row is tap row dy of output row yi - dy
+        const int64_t yo = yi - dy;
+        if (yo >= y0 && yo < y1) {
+#pragma unroll
+          for (int j = 0; j < ST_NJ; ++j) {
+            const T p = mul(wt[(dy + RY) * KW + dx + RX], e[j]);
+            acc[k][j] = (dy == -RY && dx == -RX) ? p : add(acc[k][j], p);
+          }
+        }
+      }
+    }
+    // output row yi - RY has all its taps
+    const int64_t yo = yi - RY;
+    if (yo >= y0 && yo < y1) {
+#pragma unroll
+      for (int j = 0; j < ST_NJ; ++j)
+        if (live[j]) gstore(out + yo * Wd + c0 + lane + 64 * j, acc[0][j]);
+    }
+#pragma unroll
+    for (int k = 0; k + 1 < KH; ++k)
+#pragma unroll
+      for (int j = 0; j < ST_NJ; ++j) acc[k][j] = acc[k + 1][j];
+  }
+}
+
+
+
 template <int BLOCK, int K>
 __device__ __forceinline__ void batch_run(const BatchDesc& d, uint64_t* st, uint32_t r, uint32_t epoch,
                                           unsigned char* s_raw, unsigned char* s_dyn, SegTable* s_seg) {
@@ -452,6 +583,24 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchArgs a) {
   block_retire(a.ctl, gridDim.x, epoch);
 }
 
+// The stencil problems of a batch: one workgroup per run (stencil runs never
+// wait on one another, so there are no tickets), launched after the
+// persistent kernel on the same stream.  A kernel of its own keeps the
+// stencil's registers out of the persistent kernel's budget (in it they cost
+// a wave per SIMD of occupancy for the smaller tiles, measured).
+constexpr int ST_BLOCK = 256;
+
+__global__ __launch_bounds__(ST_BLOCK) void k_batch_stencil(const BatchDesc* desc, const uint32_t* run_prob) {
+  const uint32_t g = blockIdx.x;
+  const BatchDesc d = desc[run_prob[g]];
+  switch ((d.cls & 7) >> 1) {
+    case 0: stencil_run<int32_t, 1, 1, ST_BLOCK>(d, g - d.run0); break;
+    case 1: stencil_run<int64_t, 1, 1, ST_BLOCK>(d, g - d.run0); break;
+    case 2: stencil_run<float, 1, 1, ST_BLOCK>(d, g - d.run0); break;
+    default: stencil_run<double, 1, 1, ST_BLOCK>(d, g - d.run0); break;
+  }
+}
+
 // (workgroup size, vectors per thread) instantiations offered to the tuner
 struct BatchShape {
   int block, k;
@@ -484,10 +633,25 @@ int batch_validate(const acc_batch_problem* problems, int64_t count) {
   for (int64_t i = 0; i < count; ++i) {
     const acc_batch_problem& p = problems[i];
     const long long li = (long long)i;
-    if (p.kind < ACC_BATCH_SCAN || p.kind > ACC_BATCH_HISTOGRAM)
+    if (p.kind < ACC_BATCH_SCAN || p.kind > ACC_BATCH_STENCIL)
       return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: kind %d", li, p.kind);
     if (type_index((acc_type)p.type) < 0)
       return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: element type %d", li, p.type);
+    if (p.kind == ACC_BATCH_STENCIL) {
+      const int ry = p.params & 0xff, rx = (p.params >> 8) & 0xff, bd = (p.params >> 16) & 0xff;
+      if (ry != 1 || rx != 1) return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: stencils of 3 x 3", li);
+      if (bd != ACC_BOUNDARY_CLAMP && bd != ACC_BOUNDARY_MIRROR && bd != ACC_BOUNDARY_WRAP &&
+          bd != ACC_BOUNDARY_CONSTANT)
+        return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: boundary %d", li, bd);
+      if (p.n <= 0 || p.m <= 0) return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: empty image", li);
+      if (p.n > ((int64_t)1 << 31) / ST_ROWS || p.m >= ((int64_t)1 << 30))
+        return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: image too large", li);
+      if (!p.in || !p.out || !p.aux) return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: null array", li);
+      if ((uintptr_t)p.in % type_size((acc_type)p.type) || (uintptr_t)p.out % type_size((acc_type)p.type) ||
+          (uintptr_t)p.aux % type_size((acc_type)p.type))
+        return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: arrays not element-aligned", li);
+      continue;
+    }
     if (p.kind == ACC_BATCH_HISTOGRAM ? p.op != ACC_OP_ADD : (p.op != ACC_OP_ADD && p.op != ACC_OP_MAX))
       return fail(ACC_ERROR_UNSUPPORTED, "batch: problem %lld: combine %d", li, p.op);
     if (p.n < 0 || p.m < 0) return fail(ACC_ERROR_INVALID_VALUE, "batch: problem %lld: negative size", li);
@@ -519,6 +683,7 @@ int batch_validate(const acc_batch_problem* problems, int64_t count) {
 int batch_plan(const acc_batch_problem* problems, int64_t count, int block, int k, int R, BatchPlan* plan) {
   plan->desc.assign(count, BatchDescBytes{});
   plan->run_prob.clear();
+  plan->stencil_prob.clear();
   uint64_t words = 0, runs = 0;
   size_t lds = 0;
   for (int64_t i = 0; i < count; ++i) {
@@ -552,6 +717,19 @@ int batch_plan(const acc_batch_problem* problems, int64_t count, int block, int 
     } else if (p.kind == ACC_BATCH_FOLD_SEG) {
       d.ntiles = (uint32_t)p.m;
       nr = (uint64_t)((p.m + SEG_RUN - 1) / SEG_RUN);
+    } else if (p.kind == ACC_BATCH_STENCIL) {
+      // runs of the stencil kernel: column strips x row blocks
+      const int64_t strips = (p.m + ST_STRIP - 1) / ST_STRIP;
+      const int64_t rows = (int64_t)(ST_BLOCK / 64) * ST_ROWS;
+      const uint64_t ns = (uint64_t)(strips * ((p.n + rows - 1) / rows));
+      d.ntiles = (uint32_t)strips;
+      d.R = (uint32_t)((p.params >> 16) & 0xff);  // boundary
+      d.cls = p.kind * 8 + type_index((acc_type)p.type) * 2;
+      d.run0 = (uint32_t)plan->stencil_prob.size();
+      if (d.run0 + ns > 0x7fffffffULL) return fail(ACC_ERROR_UNSUPPORTED, "batch: too many stencil runs");
+      for (uint64_t r = 0; r < ns; ++r) plan->stencil_prob.push_back((uint32_t)i);
+      memcpy(&plan->desc[i], &d, sizeof d);
+      continue;
     } else {
       // runs over index positions relative to the 16-byte boundary below the
       // indices; run 0 also writes the initial bins (so there is one even
@@ -578,8 +756,13 @@ int batch_plan(const acc_batch_problem* problems, int64_t count, int block, int 
   return ACC_SUCCESS;
 }
 
-int batch_launch_plan(int device, const BatchPlan& plan, const void* d_desc, const void* d_map, void* d_ws,
-                      hipStream_t stream) {
+int batch_launch_plan(int device, const BatchPlan& plan, const void* d_desc, const void* d_map,
+                      const void* d_smap, void* d_ws, hipStream_t stream) {
+  if (!plan.stencil_prob.empty()) {
+    hipLaunchKernelGGL(k_batch_stencil, dim3((unsigned)plan.stencil_prob.size()), dim3(ST_BLOCK), 0, stream,
+                       (const BatchDesc*)d_desc, (const uint32_t*)d_smap);
+    ACC_HIP_CHECK(hipGetLastError());
+  }
   if (!plan.total_runs) return ACC_SUCCESS;
   const DeviceInfo* di;
   ACC_CHECK(device_info(device, &di));
@@ -614,9 +797,11 @@ using namespace acc;
 struct acc_batch_s {
   int device = 0;
   std::vector<acc_batch_problem> problems;
+  std::vector<std::vector<unsigned char>> weights;  // host copies of stencil weights (the plain path's argument)
   BatchPlan plan;
   void* d_desc = nullptr;
   void* d_map = nullptr;
+  void* d_smap = nullptr;  // stencil runs -> problem
   void* d_ws = nullptr;  // SyncCtl + status words
   size_t ws_bytes = 0;
 };
@@ -626,8 +811,9 @@ namespace acc {
 static void batch_free(acc_batch_t b) {
   if (b->d_desc) dev_free(b->device, b->d_desc);
   if (b->d_map) dev_free(b->device, b->d_map);
+  if (b->d_smap) dev_free(b->device, b->d_smap);
   if (b->d_ws) dev_free(b->device, b->d_ws);
-  b->d_desc = b->d_map = b->d_ws = nullptr;
+  b->d_desc = b->d_map = b->d_smap = b->d_ws = nullptr;
 }
 
 // Lay out runs for a shape and run length; upload descriptors, map and a
@@ -638,6 +824,10 @@ static int batch_layout(acc_batch_t b, int block, int k, int R) {
   const size_t n = b->problems.size();
   ACC_CHECK(dev_malloc(b->device, std::max<size_t>(n, 1) * sizeof(BatchDescBytes), &b->d_desc));
   ACC_CHECK(dev_malloc(b->device, std::max<size_t>(b->plan.run_prob.size(), 1) * sizeof(uint32_t), &b->d_map));
+  ACC_CHECK(dev_malloc(b->device, std::max<size_t>(b->plan.stencil_prob.size(), 1) * sizeof(uint32_t), &b->d_smap));
+  if (!b->plan.stencil_prob.empty())
+    ACC_HIP_CHECK(hipMemcpy(b->d_smap, b->plan.stencil_prob.data(), b->plan.stencil_prob.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice));
   b->ws_bytes = kSyncCtlBytes + b->plan.status_words * 8;
   ACC_CHECK(dev_malloc(b->device, b->ws_bytes, &b->d_ws));
   ACC_HIP_CHECK(hipMemcpy(b->d_desc, b->plan.desc.data(), n * sizeof(BatchDescBytes), hipMemcpyHostToDevice));
@@ -652,7 +842,7 @@ static int batch_layout(acc_batch_t b, int block, int k, int R) {
 }
 
 static int batch_enqueue(acc_batch_t b, hipStream_t stream) {
-  return batch_launch_plan(b->device, b->plan, b->d_desc, b->d_map, b->d_ws, stream);
+  return batch_launch_plan(b->device, b->plan, b->d_desc, b->d_map, b->d_smap, b->d_ws, stream);
 }
 
 // Signature of a batch for the tuning cache: problem count, size histogram
@@ -766,7 +956,17 @@ int acc_hip_batch_create(int device, const acc_batch_problem* problems, int64_t 
   auto b = new acc_batch_s();
   b->device = device;
   b->problems.assign(problems, problems + count);
-  int rc = tile ? batch_layout(b, 256, tile, 4) : batch_autotune(b);
+  b->weights.resize(count);
+  int rc = ACC_SUCCESS;
+  for (int64_t i = 0; i < count && !rc; ++i) {
+    const acc_batch_problem& p = problems[i];
+    if (p.kind != ACC_BATCH_STENCIL) continue;
+    const int ry = p.params & 0xff, rx = (p.params >> 8) & 0xff;
+    b->weights[i].resize((size_t)(2 * ry + 1) * (2 * rx + 1) * type_size((acc_type)p.type));
+    const hipError_t e = hipMemcpy(b->weights[i].data(), p.aux, b->weights[i].size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "batch: reading stencil weights");
+  }
+  if (!rc) rc = tile ? batch_layout(b, 256, tile, 4) : batch_autotune(b);
   if (rc) {
     batch_free(b);
     delete b;
@@ -799,6 +999,12 @@ int acc_hip_batch_launch_plain(acc_batch_t b, acc_stream_t s) {
       case ACC_BATCH_FOLD_SEG:
         ACC_CHECK(acc_hip_fold_seg(s, t, (acc_op)p.op, seed, p.in, 1, p.n, (acc_type)p.aux_type, p.aux, p.m, p.out));
         break;
+      case ACC_BATCH_STENCIL: {
+        const uint64_t fill = p.fill;
+        ACC_CHECK(acc_hip_stencil2d(s, t, p.params & 0xff, (p.params >> 8) & 0xff, b->weights[&p - &b->problems[0]].data(),
+                                    (acc_boundary)((p.params >> 16) & 0xff), &fill, p.in, p.n, p.m, p.out));
+        break;
+      }
       default:
         ACC_CHECK(acc_hip_fill_async(s, p.out, &p.seed, type_size(t), (size_t)p.m));
         ACC_CHECK(acc_hip_permute(s, t, ACC_OP_ADD, p.out, p.out, p.m, nullptr, &p.fill, (acc_type)p.aux_type, p.in,

@@ -559,9 +559,12 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
     return acc_hip_fill_async(s, out, seed, sizeof(T), (size_t)nseg);
   }
   if (ntiles >= 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "foldSeg: too many elements");
-  // workspace: per-tile carry records, then (offsets) segment ranges and starts
+  // untagged scratch: per-tile carry records, then (offsets) segment ranges
+  // and starts; the workspace holds only the fix-up's tagged status words
+  // (a scan on the same stream reads that region by tag)
   SyncCtl* ctl;
-  uint64_t* ws;
+  uint64_t* fix_status;
+  void* scr;
   auto al256 = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t cv_bytes = al256((size_t)ntiles * sizeof(T)), cf_bytes = al256((size_t)ntiles);
   const size_t cg_bytes = (size_t)ntiles * 8;
@@ -571,9 +574,10 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
   constexpr int FIX = 256;  // tiles per k_seg_fix workgroup
   const int64_t nfix = ceil_div(ntiles, (int64_t)FIX);
   const size_t fix_bytes = (size_t)nfix * Tagged<SegV<T>>::NW * 8;
-  ACC_CHECK(stream_workspace(s, carry_bytes + segs_bytes + slot_bytes + fix_bytes, &ctl, &ws));
-  uint64_t* fix_status = (uint64_t*)((char*)ws + carry_bytes + segs_bytes + slot_bytes);
-  char* cb = (char*)ws;
+  ACC_CHECK(stream_workspace(s, fix_bytes, &ctl, &fix_status));
+  ACC_CHECK(stream_scratch(s, carry_bytes + segs_bytes + slot_bytes, &scr));
+  char* ws = (char*)scr;
+  char* cb = ws;
   const CarryArrays<T> carry{(T*)cb, (uint8_t*)(cb + cv_bytes), (T*)(cb + cv_bytes + cf_bytes),
                              (int64_t*)(cb + 2 * cv_bytes + cf_bytes)};
   const bool fill_empty = seed && check_empty;

@@ -66,6 +66,7 @@ struct acc_stream_s {
   hipStream_t stream = nullptr;
   std::mutex mu;
   std::shared_ptr<acc::Workspace> ws;
+  std::shared_ptr<acc::Workspace> scr;  // untagged scratch (status_bytes = its capacity)
   bool capturing = false;
   // workspaces outgrown by this stream: released when the stream is destroyed
   // (hipFree would synchronise the whole device at the moment of growth)
@@ -87,7 +88,8 @@ struct acc_graph_s {
   uint64_t stream_id = 0;  // the capture stream: the only stream it may be launched on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::shared_ptr<acc::Workspace> ws;  // the capture stream's workspace, kept alive
+  std::shared_ptr<acc::Workspace> ws;   // the capture stream's workspace, kept alive
+  std::shared_ptr<acc::Workspace> scr;  // and its untagged scratch
 };
 
 namespace acc {
@@ -96,13 +98,23 @@ namespace acc {
 // device.  Returns the SyncCtl pointer and the status region.  While a
 // ScratchOverride is alive on the calling thread, the override's memory is
 // returned instead (nothing is allocated) or, if it is too small, an error.
+//
+// The status region holds tagged words only (look-back status, published
+// partials): every skeleton on the stream reads it by tag, and a word of
+// arbitrary data there can carry the current tag by chance.  Data without
+// tags (per-tile records, index tables, locks) goes to stream_scratch.
 int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status);
 
-// Routes the skeleton workspace of the calling thread's launches to a
-// caller-provided device region [SyncCtl (256 B) | status words] for the
-// lifetime of the object (the allocation-free entry points).
+// Untagged scratch of `s` (at least `bytes`, 256-byte aligned, contents
+// undefined); sets the device.  Same override and capture rules as the
+// workspace.
+int stream_scratch(acc_stream_t s, size_t bytes, void** p);
+
+// Routes the skeleton workspace ([SyncCtl (256 B) | status words]) and the
+// untagged scratch of the calling thread's launches to caller-provided device
+// regions for the lifetime of the object (the allocation-free entry points).
 struct ScratchOverride {
-  ScratchOverride(int device, void* base, size_t bytes);
+  ScratchOverride(int device, void* ws, size_t ws_bytes, void* scratch, size_t scratch_bytes);
   ~ScratchOverride();
   ScratchOverride(const ScratchOverride&) = delete;
   ScratchOverride& operator=(const ScratchOverride&) = delete;

@@ -717,10 +717,10 @@ static int run(acc_kernel_t k, acc_stream_t s, const acc_kernel_args* ka) {
       if (n == 0 || m == 0) return ACC_SUCCESS;
       if (!in_ok) ACC_CHECK(check_fields(ka->in, nd.elt, "permute source"));
       if (!nd.cas) {
-        acc::SyncCtl* ctl;
-        uint64_t* st;
-        ACC_CHECK(stream_workspace(s, (size_t)m * 4, &ctl, &st));
-        a.locks = (uint32_t*)st;
+        // the locks are untagged words: stream scratch, not the status region
+        void* lk;
+        ACC_CHECK(stream_scratch(s, (size_t)m * 4, &lk));
+        a.locks = (uint32_t*)lk;
         const uint32_t zero = 0;
         // a fill kernel, not a memset node (kept capturable)
         ACC_CHECK(acc_hip_fill_async(s, a.locks, &zero, 4, (size_t)m));

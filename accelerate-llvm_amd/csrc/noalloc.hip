@@ -1,7 +1,8 @@
 // Allocation-free foldSeg / fold1Seg: the same kernels as acc_hip_fold_seg,
-// with their workspace (the launch epoch and tickets, per-tile carry records,
-// per-tile segment ranges and starts, fix-up status words) in a fixed device
-// array of this file instead of the stream's allocated workspace.  No device
+// with their workspace (the launch epoch and tickets, the fix-up's status
+// words) and untagged scratch (per-tile carry records, per-tile segment
+// ranges and starts) in a fixed device array of this file instead of the
+// stream's allocations.  No device
 // memory is allocated or freed by a call, so callers that cannot allocate
 // (code under graph capture, tight serving loops, harnesses that forbid
 // hipMalloc) can use it.
@@ -16,9 +17,12 @@
 
 namespace acc {
 
-// 16 MiB: a fold1Seg over about 2^28.9 Float32 (or 2^27.9 Float64) elements
-// in any number of segments; larger calls are refused before any launch.
+// 16 MiB: the first MiB is the workspace (launch epoch, tickets, tagged
+// status words), the rest the untagged per-tile records and tables.  Enough
+// for a fold1Seg over about 2^28.7 Float32 (or 2^27.7 Float64) elements in
+// any number of segments; larger calls are refused before any launch.
 constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+constexpr size_t kNoAllocWorkspaceBytes = (size_t)1 << 20;
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -52,6 +56,7 @@ extern "C" int acc_hip_fold_seg_noalloc(acc_stream_t s, acc_type t, acc_op op, c
   if (!s) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (no-alloc): null stream");
   void* base = nullptr;
   ACC_CHECK(scratch_address(s->device, &base));
-  ScratchOverride guard(s->device, base, kNoAllocScratchBytes);
+  ScratchOverride guard(s->device, base, kNoAllocWorkspaceBytes, (char*)base + kNoAllocWorkspaceBytes,
+                        kNoAllocScratchBytes - kNoAllocWorkspaceBytes);
   return acc_hip_fold_seg(s, t, op, seed, in, outer, inner, offset_type, offsets, nseg, out);
 }

@@ -149,12 +149,16 @@ Workspace::~Workspace() {
 
 struct ScratchRegion {
   int device = -1;
-  void* base = nullptr;
+  void* base = nullptr;  // workspace
   size_t bytes = 0;
+  void* scr = nullptr;   // untagged scratch
+  size_t scr_bytes = 0;
 };
 static thread_local ScratchRegion tl_scratch;
 
-ScratchOverride::ScratchOverride(int device, void* base, size_t bytes) { tl_scratch = {device, base, bytes}; }
+ScratchOverride::ScratchOverride(int device, void* ws, size_t ws_bytes, void* scratch, size_t scratch_bytes) {
+  tl_scratch = {device, ws, ws_bytes, scratch, scratch_bytes};
+}
 ScratchOverride::~ScratchOverride() { tl_scratch = ScratchRegion{}; }
 
 int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_t** status) {
@@ -193,6 +197,39 @@ int stream_workspace(acc_stream_t s, size_t status_bytes, SyncCtl** ctl, uint64_
   }
   *ctl = (SyncCtl*)s->ws->ptr;
   *status = (uint64_t*)((char*)s->ws->ptr + kCtlBytes);
+  return ACC_SUCCESS;
+}
+
+int stream_scratch(acc_stream_t s, size_t bytes, void** p) {
+  if (!s) return fail(ACC_ERROR_INVALID_VALUE, "null stream");
+  ACC_CHECK(set_device(s->device));
+  if (tl_scratch.base) {
+    if (tl_scratch.device != s->device)
+      return fail(ACC_ERROR_INVALID_VALUE, "scratch of device %d used on device %d", tl_scratch.device, s->device);
+    if (bytes > tl_scratch.scr_bytes)
+      return fail(ACC_ERROR_INVALID_VALUE,
+                  "allocation-free call needs %zu bytes of untagged scratch; the maximum is %zu bytes", bytes,
+                  tl_scratch.scr_bytes);
+    *p = tl_scratch.scr;
+    return ACC_SUCCESS;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (!s->scr || bytes > s->scr->status_bytes) {
+    if (s->capturing)
+      return fail(ACC_ERROR_INVALID_VALUE,
+                  "stream scratch must grow to %zu bytes during capture; call acc_hip_stream_reserve before "
+                  "capturing", bytes);
+    size_t cap = s->scr ? s->scr->status_bytes : (size_t)1 << 20;
+    while (cap < bytes) cap *= 2;
+    auto w = std::make_shared<Workspace>();
+    w->device = s->device;
+    ACC_CHECK(dev_malloc(s->device, cap, &w->ptr));
+    w->status_bytes = cap;
+    // (queued work may still use the old one: retired, not freed)
+    if (s->scr) s->retired.push_back(s->scr);
+    s->scr = w;
+  }
+  *p = s->scr->ptr;
   return ACC_SUCCESS;
 }
 
@@ -415,6 +452,7 @@ int acc_hip_stream_destroy(acc_stream_t s) {
     (void)hipStreamSynchronize(s->stream);
     // a workspace that a live graph still references outlives the stream
     s->ws.reset();
+    s->scr.reset();
     s->retired.clear();
     (void)hipStreamDestroy(s->stream);
   }
@@ -446,7 +484,9 @@ int acc_hip_stream_device(acc_stream_t s, int* device) {
 int acc_hip_stream_reserve(acc_stream_t s, size_t status_bytes) {
   SyncCtl* c;
   uint64_t* st;
-  return stream_workspace(s, status_bytes, &c, &st);
+  void* p;
+  ACC_CHECK(stream_workspace(s, status_bytes, &c, &st));
+  return stream_scratch(s, status_bytes, &p);
 }
 
 int acc_hip_event_create(int device, int timing, acc_event_t* out) {
@@ -568,7 +608,8 @@ int acc_hip_graph_end_capture(acc_stream_t s, acc_graph_t* out) {
   g->stream_id = s->id;
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    g->ws = s->ws;  // growth is refused during capture: this is the one the kernels captured
+    g->ws = s->ws;  // growth is refused during capture: these are the ones the kernels captured
+    g->scr = s->scr;
   }
   ACC_HIP_CHECK(hipStreamEndCapture(s->stream, &g->graph));
   hipError_t e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);

@@ -5,14 +5,15 @@
 // result being complete on the device.
 //
 // Continuous batching: there is no fixed batch boundary.  A slot that admits
-// a scan, fold, foldSeg or histogram also admits every other such request
-// that has arrived and whose output fits, up to a round limit, and runs the
-// whole round as ONE launch of the batched form's persistent kernel (one
-// descriptor upload + one launch per round instead of one or more launches
-// per request); requests that arrive while a round executes form the next
-// round on another slot, so rounds overlap on the GPU.  Stencils and the
-// combinations the batched kernel does not take run through the plain entry
-// points on their slot.  Admission is by free HBM: outputs are carved from a
+// a scan, fold, foldSeg, histogram or stencil also admits every other such
+// request that has arrived and whose output fits, up to a round limit, and
+// runs the whole round as ONE launch of the batched form's persistent kernel
+// (plus one of its stencil kernel when the round holds stencils: one upload
+// of descriptors, run maps and stencil weights per round instead of one or
+// more launches per request); requests that arrive
+// while a round executes form the next round on another slot, so rounds
+// overlap on the GPU.  The combinations the batched kernel does not take run
+// through the plain entry points on their slot.  Admission is by free HBM: outputs are carved from a
 // device arena sized at creation from hipMemGetInfo (or the caller's budget,
 // whichever is smaller); a request whose output does not fit waits, and later
 // small requests may pass it.
@@ -202,9 +203,11 @@ static int run_request(acc_server_t s, Req& q, acc_stream_t st, void* out) {
 }
 
 // The batched kernel's form of a request, if it has one: left-to-right
-// scans, folds and foldSegs with (+) or max, and (+) permutes of a constant
-// source into at most 4096 bins, over Int32/Int64/Float32/Float64, with the
-// alignment the batched kernel needs.
+// scans, folds and foldSegs with (+) or max, (+) permutes of a constant
+// source into at most 4096 bins, and 3 x 3 stencils with a clamp,
+// mirror, wrap or constant boundary, over Int32/Int64/Float32/Float64, with
+// the alignment the batched kernel needs.  A stencil's weights pointer is
+// filled in by the round (they travel with its descriptors).
 static bool batchable(const acc_request& r, void* out, acc_batch_problem* p) {
   if (r.type != ACC_INT32 && r.type != ACC_INT64 && r.type != ACC_FLOAT32 && r.type != ACC_FLOAT64) return false;
   const bool add_max = r.op == ACC_OP_ADD || r.op == ACC_OP_MAX;
@@ -232,6 +235,20 @@ static bool batchable(const acc_request& r, void* out, acc_batch_problem* p) {
       p->aux_type = r.aux_type;
       p->m = r.m;
       return true;
+    case ACC_REQ_STENCIL2D: {
+      const size_t B = type_size((acc_type)r.type);
+      if (r.ry != 1 || r.rx != 1 || r.n <= 0 || r.m <= 0 || !r.in) return false;
+      if (r.boundary != ACC_BOUNDARY_CLAMP && r.boundary != ACC_BOUNDARY_MIRROR && r.boundary != ACC_BOUNDARY_WRAP &&
+          r.boundary != ACC_BOUNDARY_CONSTANT)
+        return false;
+      if ((uintptr_t)r.in % B || (uintptr_t)out % B || r.n > ((int64_t)1 << 26) || r.m > ((int64_t)1 << 30)) return false;
+      p->kind = ACC_BATCH_STENCIL;
+      p->in = r.in;
+      p->m = r.m;
+      p->params = r.ry | (r.rx << 8) | (r.boundary << 16);
+      p->fill = r.fill;
+      return true;
+    }
     case ACC_REQ_PERMUTE:
       if (r.op != ACC_OP_ADD || r.in || r.m <= 0 || r.m > 4096 || !idx_ok || r.n < 0) return false;
       p->kind = ACC_BATCH_HISTOGRAM;
@@ -267,29 +284,55 @@ static int grow_device(int device, void** p, size_t* cap, size_t need) {
   return ACC_SUCCESS;
 }
 
-// One fused round on the slot's stream: plan, one upload of descriptors and
-// run map, one launch.  The slot's previous round has completed, so its
-// buffers can be reused (or replaced) freely.
-static int run_round(acc_server_t s, int slot, const std::vector<acc_batch_problem>& probs, BatchPlan& plan) {
+// One fused round on the slot's stream: plan, one upload of stencil weights,
+// descriptors and run maps, one launch of the persistent kernel (and one of
+// the stencil kernel when the round holds stencils).  The slot's previous round has
+// completed, so its buffers can be reused (or replaced) freely.
+static int run_round(acc_server_t s, int slot, std::vector<acc_batch_problem>& probs,
+                     const std::vector<std::shared_ptr<Req>>& round, BatchPlan& plan) {
   const acc_server_config& c = s->cfg;
   RoundBufs& b = s->rounds_buf[slot];
   hipStream_t stream = s->slots[slot]->stream;
-  ACC_CHECK(batch_plan(probs.data(), (int64_t)probs.size(), c.block, c.vectors, c.run, &plan));
+  // the weights go first, at offsets known before planning
+  std::vector<size_t> w_off(probs.size(), 0);
+  size_t w_bytes = 0;
+  for (size_t i = 0; i < probs.size(); ++i) {
+    if (probs[i].kind != ACC_BATCH_STENCIL) continue;
+    w_off[i] = w_bytes;
+    w_bytes += (round[i]->weights.size() + 15) / 16 * 16;
+  }
+  w_bytes = (w_bytes + 255) / 256 * 256;
   const size_t desc_bytes = (probs.size() * sizeof(BatchDescBytes) + 255) / 256 * 256;
-  const size_t map_bytes = plan.run_prob.size() * sizeof(uint32_t);
-  const size_t bytes = desc_bytes + map_bytes;
+  size_t bytes = 0, map_bytes = 0, smap_bytes = 0;
+  // plan against the device buffer's address; growing it moves the weights,
+  // so the plan is made again (at most once)
+  for (int pass = 0; pass < 3; ++pass) {
+    for (size_t i = 0; i < probs.size(); ++i)
+      if (probs[i].kind == ACC_BATCH_STENCIL) probs[i].aux = (char*)b.d + w_off[i];
+    ACC_CHECK(batch_plan(probs.data(), (int64_t)probs.size(), c.block, c.vectors, c.run, &plan));
+    map_bytes = (plan.run_prob.size() * sizeof(uint32_t) + 255) / 256 * 256;
+    smap_bytes = plan.stencil_prob.size() * sizeof(uint32_t);
+    bytes = w_bytes + desc_bytes + map_bytes + smap_bytes;
+    if (b.d && bytes <= b.d_cap) break;
+    ACC_CHECK(grow_device(s->device, &b.d, &b.d_cap, bytes));
+  }
   ACC_CHECK(grow_host(&b.h, &b.h_cap, bytes));
-  ACC_CHECK(grow_device(s->device, &b.d, &b.d_cap, bytes));
   const size_t ws_bytes = kSyncCtlBytes + plan.status_words * 8;
   if (ws_bytes > b.ws_cap) {
     ACC_CHECK(grow_device(s->device, &b.ws, &b.ws_cap, ws_bytes));
     // fresh memory: no stale word may carry the current epoch's tag
     ACC_HIP_CHECK(hipMemsetAsync(b.ws, 0, b.ws_cap, stream));
   }
-  memcpy(b.h, plan.desc.data(), probs.size() * sizeof(BatchDescBytes));
-  memcpy((char*)b.h + desc_bytes, plan.run_prob.data(), map_bytes);
+  char* h = (char*)b.h;
+  for (size_t i = 0; i < probs.size(); ++i)
+    if (probs[i].kind == ACC_BATCH_STENCIL) memcpy(h + w_off[i], round[i]->weights.data(), round[i]->weights.size());
+  memcpy(h + w_bytes, plan.desc.data(), probs.size() * sizeof(BatchDescBytes));
+  memcpy(h + w_bytes + desc_bytes, plan.run_prob.data(), plan.run_prob.size() * sizeof(uint32_t));
+  memcpy(h + w_bytes + desc_bytes + map_bytes, plan.stencil_prob.data(), smap_bytes);
   ACC_HIP_CHECK(hipMemcpyAsync(b.d, b.h, bytes, hipMemcpyHostToDevice, stream));
-  return batch_launch_plan(s->device, plan, b.d, (char*)b.d + desc_bytes, b.ws, stream);
+  char* d = (char*)b.d;
+  return batch_launch_plan(s->device, plan, d + w_bytes, d + w_bytes + desc_bytes, d + w_bytes + desc_bytes + map_bytes,
+                           b.ws, stream);
 }
 
 // One worker thread per slot: it takes the earliest arrived request whose
@@ -332,9 +375,14 @@ static void slot_worker(acc_server_t s, int slot) {
         continue;
       }
       void* out = (char*)s->arena_base + off;
+      const bool fused = fusable && batchable(q->r, out, &bp);
+      if (!fused && !round.empty()) {  // (arena offsets keep every alignment the check above assumed)
+        s->arena.release(off, q->out_bytes);
+        ++i;
+        continue;
+      }
       q->out_off = off;
       q->slot = slot;
-      const bool fused = fusable && batchable(q->r, out, &bp);
       round.push_back(q);
       s->pending.erase(s->pending.begin() + (std::ptrdiff_t)i);
       if (!fused) break;  // a plain request runs alone
@@ -349,7 +397,7 @@ static void slot_worker(acc_server_t s, int slot) {
         Req& q = *round.front();
         rc = run_request(s, q, st, (char*)s->arena_base + q.out_off);
       } else {
-        rc = run_round(s, slot, probs, plan);
+        rc = run_round(s, slot, probs, round, plan);
       }
       if (!rc && hipEventRecord(ev, st->stream) != hipSuccess) rc = ACC_ERROR_HIP;
       while (!rc) {  // poll: a blocking wait adds the scheduler's wake-up latency
@@ -432,6 +480,15 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
     // skeleton workspace up front (enough for foldSeg over ~2^29 elements),
     // so that no request grows it while others are in flight
     ACC_CHECK(acc_hip_stream_reserve(st, (size_t)16 << 20));
+    // round buffers up front (host staging and device copy of a round's
+    // weights, descriptors and run map; the batched kernel's workspace,
+    // zeroed now): rounds of up to ~2 MiB of those allocate nothing, where a
+    // pinned host allocation alone costs ~0.1-0.3 ms
+    RoundBufs& rb = s->rounds_buf[k];
+    ACC_CHECK(grow_host(&rb.h, &rb.h_cap, (size_t)1 << 20));
+    ACC_CHECK(grow_device(device, &rb.d, &rb.d_cap, (size_t)1 << 20));
+    ACC_CHECK(grow_device(device, &rb.ws, &rb.ws_cap, (size_t)1 << 20));
+    ACC_HIP_CHECK(hipMemset(rb.ws, 0, rb.ws_cap));
     hipEvent_t e;
     ACC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s->slot_event.push_back(e);

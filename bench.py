@@ -97,6 +97,7 @@ def bench_batched(s, args):
             else:
                 A.fold(s, op, seed, src, 1, n, out)
 
+    log("batched: inputs on the device; per-call loop from Python")
     per_call_python()
     s.synchronize()
     e0, e1 = A.Event(0), A.Event(0)
@@ -108,15 +109,18 @@ def bench_batched(s, args):
     percall_py_wall = time.perf_counter() - w0
     percall_py_ms = e0.elapsed_ms(e1)
     ref_fold = d_yf.to_numpy(s)[::64].copy()
+    log("batched: creating the batch (autotuned shape)")
     t0 = time.perf_counter()
     b = A.Batch(0, probs, tile=0)
     tune_s = time.perf_counter() - t0
+    log(f"batched: shape {b.shape()} chosen in {tune_s:.2f} s; timing the per-call path from C")
     k, tiles = b.tile()
     shape = b.shape()
     # the per-call path from C: one plain entry-point call per problem, in
     # order on the stream (no Python between the calls)
     percall_ms = timed(s, lambda: b.launch_plain(s), reps=5, warmup=1)
     assert np.array_equal(d_yf.to_numpy(s)[::64], ref_fold), "plain calls from C differ from the Python loop"
+    log("batched: capturing and checking the batched launch")
     g = A.Graph.capture(s, lambda: b.launch(s))
     g.launch(s)
     s.synchronize()
@@ -219,7 +223,9 @@ def bench_serving(s, args):
         srv.reset_stats()
         return st
 
+    log(f"serving: {R} requests; plain per-call path (one slot, no fusion)")
     plain_srv = A.Server(0, slots=1, fuse=False)
+    capacity_of(plain_srv)  # untimed pass: every kernel of the mix loaded and warm
     plain = capacity_of(plain_srv).req_per_s
     plain_srv.destroy()
     # and from Python (ctypes call + synchronize per request), for reference
@@ -230,6 +236,7 @@ def bench_serving(s, args):
         s.synchronize()
     plain_py = R / (time.perf_counter() - plain_t0)
 
+    log(f"serving: plain {plain:.0f} req/s; unfused and fused servers")
     unfused_srv = A.Server(0, slots=args.serve_slots, fuse=False)
     unfused = capacity_of(unfused_srv).req_per_s
     unfused_srv.destroy()

@@ -155,8 +155,9 @@ int acc_hip_stream_destroy(acc_stream_t s);
 int acc_hip_stream_synchronize(acc_stream_t s);
 int acc_hip_stream_query(acc_stream_t s); /* ACC_SUCCESS = idle, ACC_ERROR_NOT_READY = busy */
 int acc_hip_stream_device(acc_stream_t s, int* device);
-/* Pre-size the stream's skeleton workspace (bytes of tile-status words), so that
- * no allocation happens later, e.g. while the stream is being captured. */
+/* Pre-size the stream's skeleton workspace (bytes of tile-status words, and as
+ * many bytes of untagged per-tile scratch), so that no allocation happens
+ * later, e.g. while the stream is being captured. */
 int acc_hip_stream_reserve(acc_stream_t s, size_t status_bytes);
 
 /* ----------------------------------------------------------------- events
@@ -514,14 +515,20 @@ int acc_hip_table_destroy(acc_table_t t);
  *   ACC_BATCH_HISTOGRAM: permute (+) of a constant source: out = m bins, each
  *                        first `seed`, then out[in[i]] (+)= fill for i < n with
  *                        0 <= in[i] < m (in: indices of aux_type); m <= 4096
+ *   ACC_BATCH_STENCIL:   acc_hip_stencil2d on an n x m image (rows x columns):
+ *                        params = ry | rx << 8 | boundary << 16 with
+ *                        ry == rx == 1 (3 x 3) and a clamp, mirror, wrap or constant
+ *                        boundary (its value in `fill`); aux = the weights
+ *                        (device memory, (2ry+1)(2rx+1) of the element type)
  * Element types INT32, INT64, FLOAT32, FLOAT64; combine ADD or MAX (ADD for
- * histograms).  Scan and fold inputs (and scan outputs) must be 16-byte
- * aligned. */
+ * histograms; none for stencils).  Scan and fold inputs (and scan outputs)
+ * must be 16-byte aligned. */
 typedef enum acc_batch_kind {
   ACC_BATCH_SCAN = 0,
   ACC_BATCH_FOLD = 1,
   ACC_BATCH_FOLD_SEG = 2,
-  ACC_BATCH_HISTOGRAM = 3
+  ACC_BATCH_HISTOGRAM = 3,
+  ACC_BATCH_STENCIL = 4
 } acc_batch_kind;
 
 typedef struct acc_batch_problem {
@@ -533,11 +540,12 @@ typedef struct acc_batch_problem {
   const void* in;   /* values; histogram: the indices */
   void* out;
   uint64_t seed;    /* bit pattern of the seed in the low bytes (histogram: the initial bin value) */
-  const void* aux;  /* foldSeg: offsets */
-  int64_t m;        /* foldSeg: segments; histogram: bins */
+  const void* aux;  /* foldSeg: offsets; stencil: weights */
+  int64_t m;        /* foldSeg: segments; histogram: bins; stencil: columns */
   int32_t aux_type; /* foldSeg offsets, histogram indices: ACC_INT32 or ACC_INT64 */
-  int32_t reserved;
-  uint64_t fill;    /* histogram: the constant source value (bit pattern) */
+  int32_t params;   /* stencil: ry | rx << 8 | boundary << 16 */
+  uint64_t fill;    /* histogram: the constant source value; stencil: the constant
+                     * boundary's value (bit patterns) */
 } acc_batch_problem;
 
 typedef struct acc_batch_s* acc_batch_t;

@@ -1,5 +1,5 @@
-"""Batched execution: many uneven scans, folds, segmented folds and
-histograms in one persistent launch.
+"""Batched execution: many uneven scans, folds, segmented folds, histograms
+and stencils in one persistent launch.
 
 Each problem's result must equal its standalone result (integers bit-exact,
 floating sums within the FP32 bound of the FP64 truth), for every shape
@@ -184,3 +184,86 @@ def test_batch_rejects_bad_problems(stream):
         A.Batch(0, [(A.BATCH_SCAN, ADD, None, x.view(1, 9), 9, o)], tile=2)  # misaligned input
     with pytest.raises(A.AccError):
         A.Batch(0, [(A.BATCH_HISTOGRAM, ADD, 0, x, 10, o, None, 5000, 1)], tile=2)  # too many bins
+
+
+def make_stencils(stream, count, seed):
+    """3 x 3 stencils of every boundary and element type, on images from
+    1 x 1 to a few hundred rows and columns (wider than one 128-column strip,
+    taller than one run, ragged at both edges)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        dt = TYPES[i % 4]
+        r = 1
+        bnd = [A.CLAMP, A.MIRROR, A.WRAP, A.CONSTANT][(i // 4) % 4]
+        h = int(rng.integers(1, 300)) if i % 5 else int(rng.integers(1, 4))
+        w = int(rng.integers(1, 700)) if i % 7 else int(rng.integers(1, 4))
+        k = 2 * r + 1
+        if np.dtype(dt).kind == "f":
+            img = rng.standard_normal((h, w)).astype(dt)
+            wt = rng.standard_normal((k, k)).astype(dt)
+            cval = dt(rng.standard_normal())
+        else:
+            img = rng.integers(-1000, 1000, size=(h, w)).astype(dt)
+            wt = rng.integers(-9, 10, size=(k, k)).astype(dt)
+            cval = dt(rng.integers(-50, 50))
+        out.append(dict(kind=A.BATCH_STENCIL, op=ADD, seed=None, x=img, n=h, m=w, dt=dt, r=r, bnd=bnd, wt=wt,
+                        cval=cval, src=A.DeviceArray.from_numpy(stream, img),
+                        aux=A.DeviceArray.from_numpy(stream, wt.reshape(-1)), fill=cval,
+                        params=r | (r << 8) | (bnd << 16), out=A.DeviceArray(h * w, dt)))
+    return out
+
+
+def stencil_ok(p, got):
+    """Integers wrap exactly; floating results within taps * eps * sum |w x|
+    of the float64 oracle (the oracle sums in tuple order; a fused
+    multiply-add may round differently)."""
+    r, bnd, wt, img = p["r"], p["bnd"], p["wt"], p["x"]
+    want = O.stencil2d(r, r, wt, bnd, p["cval"], img).reshape(-1)
+    if np.dtype(p["dt"]).kind != "f":
+        return np.array_equal(got, want)
+    w64, x64 = wt.astype(np.float64), img.astype(np.float64)
+    truth = O.stencil2d(r, r, w64, bnd, float(p["cval"]), x64).reshape(-1)
+    ab = O.stencil2d(r, r, np.abs(w64), bnd, abs(float(p["cval"])), np.abs(x64)).reshape(-1)
+    eps = 2.0 ** -23 if p["dt"] == np.float32 else 2.0 ** -52
+    taps = (2 * r + 1) ** 2
+    return np.all(np.abs(got.astype(np.float64) - truth) <= 2 * taps * eps * ab + 1e-300)
+
+
+def stencil_tuple(p):
+    return (p["kind"], p["op"], None, p["src"], p["n"], p["out"], p["aux"], p["m"], p["fill"], p["params"])
+
+
+@pytest.mark.parametrize("tile", [2, 8])
+def test_batch_stencils_match_oracle_and_plain(stream, tile):
+    """Stencils alone and mixed with the other kinds in one batch; each result
+    against the oracle, integers also bit-exact against the plain
+    acc_hip_stencil2d call on the same problem."""
+    st = make_stencils(stream, 64, seed=100 + tile)
+    others = make_problems(stream, 40, seed=200 + tile, kinds=ALL_KINDS)
+    b = A.Batch(0, batch_of(others) + [stencil_tuple(p) for p in st], tile=tile)
+    b.launch(stream)
+    stream.synchronize()
+    mixed = [p["out"].to_numpy(stream) for p in st]
+    for p, got in zip(st, mixed):
+        assert stencil_ok(p, got), (np.dtype(p["dt"]).name, p["r"], p["bnd"], p["n"], p["m"])
+    for p in others:
+        check(stream, p, p["out"].to_numpy(stream))
+    b.launch_plain(stream)
+    stream.synchronize()
+    for p, got in zip(st, mixed):
+        if np.dtype(p["dt"]).kind != "f":
+            assert np.array_equal(got, p["out"].to_numpy(stream)), (np.dtype(p["dt"]).name, p["r"], p["bnd"])
+
+
+def test_batch_stencil_rejects_unsupported(stream):
+    x = A.DeviceArray.from_numpy(stream, np.zeros(64, np.float32))
+    o = A.DeviceArray(64, np.float32)
+    wt = A.DeviceArray.from_numpy(stream, np.ones(15, np.float32))
+    with pytest.raises(A.AccError):  # 3 x 5
+        A.Batch(0, [(A.BATCH_STENCIL, ADD, None, x, 8, o, wt, 8, 0.0, 1 | (2 << 8) | (A.CLAMP << 16))], tile=2)
+    wt5 = A.DeviceArray.from_numpy(stream, np.ones(25, np.float32))
+    with pytest.raises(A.AccError):  # 5 x 5: the plain stencil's
+        A.Batch(0, [(A.BATCH_STENCIL, ADD, None, x, 8, o, wt5, 8, 0.0, 2 | (2 << 8) | (A.CLAMP << 16))], tile=2)
+    with pytest.raises(A.AccError):  # the Function boundary
+        A.Batch(0, [(A.BATCH_STENCIL, ADD, None, x, 8, o, wt, 8, 0.0, 1 | (1 << 8) | (A.FUNCTION << 16))], tile=2)

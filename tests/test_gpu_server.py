@@ -178,11 +178,11 @@ def out_count(kind, size):
 def test_fused_rounds_same_alone_and_in_the_mix(stream, pool):
     """All requests arrive at once, so slots take them in fused rounds (one
     batched launch per round); requests without a batched form (a right-to-
-    left scan, a min fold1Seg, the stencils) run through the plain calls
-    beside them.  Every result agrees with the oracle, and each request run
-    alone (a round of its own) gives the same result: bit-exact for integers
-    and histograms, within the FP32 bound for floating (+) (the look-back
-    may associate the tile sums differently from run to run)."""
+    left scan, a min fold1Seg) run through the plain calls beside them.
+    Every result agrees with the oracle, and each request run alone (a round
+    of its own) gives the same result: bit-exact for integers, histograms and
+    stencils, within the FP32 bound for floating (+) scans and folds (the
+    look-back may associate the tile sums differently from run to run)."""
     reqs = mix(pool, 90, seed=2)
     off = pool["off_host"]
     for k, key in enumerate(("f32", "i32", "i64")):
@@ -209,7 +209,7 @@ def test_fused_rounds_same_alone_and_in_the_mix(stream, pool):
         mixed.append(got)
         srv.release(rid)
     st = srv.stats()
-    fusable = sum(k in ("scan", "fold1seg", "hist") for (k, *_) in reqs)
+    fusable = sum(k in ("scan", "fold1seg", "hist", "stencil") for (k, *_) in reqs)
     assert st.completed == len(reqs) and st.failed == 0
     assert st.fused == fusable and 1 <= st.rounds < fusable, (st.rounds, st.fused, fusable)
     srv.destroy()

@@ -22,10 +22,16 @@
 //               the tile's segmented aggregate go to a per-tile record;
 //   k_seg_fix   a segmented scan over the tile aggregates (256 tiles per
 //               workgroup, chained by a look-back), which completes the open
-//               ends (at most one per tile).
-// A single-pass look-back would chain every tile to its predecessor; with
-// segments shorter than a tile almost every tile needs a carry, and the
-// chain of round trips, not HBM, then bounds the launch.
+//               ends (at most one per tile).  (One workgroup over all the
+//               records took 16 us for 2^13 tiles, against 5: one CU cannot
+//               issue the scattered record loads fast enough, measured.)
+// No kernel waits on another workgroup.  (Completing the open ends inside
+// k_fold_seg -- tagged records, an arrival count per group of tiles, the
+// group's last tile scanning the records -- was built and measured: the
+// per-tile atomic's round trip kept every tile resident longer and the tile
+// kernel lost as much time as the fix-up launch takes; searching the offsets
+// inside the tiles instead of k_tile_seg put dependent round trips ahead of
+// the data loads and cost more.)
 //
 // Segments are given as offsets (the `scanl (+) 0` of the lengths; the
 // frontend has already applied it, PTX/Execute.hs:359,

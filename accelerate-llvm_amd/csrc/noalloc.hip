@@ -1,28 +1,28 @@
 // Allocation-free foldSeg / fold1Seg: the same kernels as acc_hip_fold_seg,
-// with their workspace (the launch epoch and tickets, the fix-up's status
-// words) and untagged scratch (per-tile carry records, per-tile segment
-// ranges and starts) in a fixed device array of this file instead of the
-// stream's allocations.  No device
+// with their workspace (the launch epoch and tickets, the fix-up's tagged
+// status words) and untagged scratch (per-tile carry records, per-tile
+// segment ranges and starts) in a fixed device array of this file instead of
+// the stream's allocations.  No device
 // memory is allocated or freed by a call, so callers that cannot allocate
 // (code under graph capture, tight serving loops, harnesses that forbid
 // hipMalloc) can use it.
 //
 // `scratch` is zero at load.  Nothing in it needs clearing between calls:
-// the tile status words carry the launch epoch, which the last workgroup of
-// every launch advances, and every per-tile record is written by one kernel
-// before the next kernel of the same call reads it.  Every call on a device
+// the status words carry the launch epoch, which the last workgroup of every
+// launch advances, and every per-tile record and table is written by one
+// kernel of a call before the next kernel of the same call reads it.  Every call on a device
 // shares that device's copy, so two calls that overlap on one device (on
 // different streams, or from different threads) race.
 #include "host.hpp"
 
 namespace acc {
 
-// 16 MiB: the first MiB is the workspace (launch epoch, tickets, tagged
-// status words), the rest the untagged per-tile records and tables.  Enough
-// for a fold1Seg over about 2^28.7 Float32 (or 2^27.7 Float64) elements in
-// any number of segments; larger calls are refused before any launch.
+// 16 MiB: the first 64 KiB are the workspace (launch epoch, tickets, the
+// fix-up's status words), the rest the records and tables of a fold1Seg over
+// about 2^28.8 Float32 (or 2^27.8 Float64) elements in any number of
+// segments; larger calls are refused before any launch.
 constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
-constexpr size_t kNoAllocWorkspaceBytes = (size_t)1 << 20;
+constexpr size_t kNoAllocWorkspaceBytes = (size_t)64 << 10;
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 

@@ -91,9 +91,17 @@ template <typename T> struct ScanTune {
   static constexpr int K_UNALIGNED = 4;  // element accesses: a smaller tile
 };
 
-template <typename T, typename Op, bool REVERSE, bool ALIGNED>
+// Scans of fewer than kSmallTiles full-size tiles (below 2^20 4-byte or
+// 2^19 8-byte elements) use 2 vectors per thread: eight times the
+// workgroups for the same data, so a short scan is spread over many CUs
+// instead of waiting on the few loads a handful of large tiles put in flight
+// (2^16 Int64, the offsets of a 2^16-segment foldSeg: 8 tiles otherwise).
+constexpr int64_t kSmallTiles = 64;
+constexpr int kSmallK = 2;
+
+template <typename T, typename Op, bool REVERSE, bool ALIGNED, int K>
 int scan_launch(acc_stream_t s, ScanArgs<T> a) {
-  constexpr int BLOCK = ScanTune<T>::BLOCK, K = ALIGNED ? ScanTune<T>::K : ScanTune<T>::K_UNALIGNED;
+  constexpr int BLOCK = ScanTune<T>::BLOCK;
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
   const bool ragged = a.io.n % TILE != 0;
   const uint32_t nfull = a.ntiles - (ragged ? 1u : 0u);
@@ -131,7 +139,9 @@ int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const 
   if ((uintptr_t)in % sizeof(T) || (uintptr_t)out % sizeof(T))
     return fail(ACC_ERROR_INVALID_VALUE, "scan: arrays not element-aligned");
   const bool aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
-  const int64_t TILE = (int64_t)BLOCK * (aligned ? ScanTune<T>::K : ScanTune<T>::K_UNALIGNED) * VEC;
+  const bool small = aligned && !reverse && ceil_div(n, (int64_t)BLOCK * ScanTune<T>::K * VEC) < kSmallTiles;
+  const int64_t TILE =
+      (int64_t)BLOCK * (small ? kSmallK : (aligned ? ScanTune<T>::K : ScanTune<T>::K_UNALIGNED)) * VEC;
   const int64_t ntiles = ceil_div(n, TILE);
   if (ntiles > 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "scan: %lld elements", (long long)n);
   SyncCtl* ctl;
@@ -140,8 +150,11 @@ int scan_all(acc_stream_t s, bool reverse, bool exclusive, const T* seed, const 
   TileIO<T> io{in, out, n, 0u, (uint32_t)(ntiles - 1), exclusive ? 1u : 0u, seed ? 1u : 0u, seed ? *seed : T(),
                total_out, seed_out, status};
   ScanArgs<T> a{io, (uint32_t)ntiles, 0u, 0u, ctl};
-  if (reverse) return aligned ? scan_launch<T, Op, true, true>(s, a) : scan_launch<T, Op, true, false>(s, a);
-  return aligned ? scan_launch<T, Op, false, true>(s, a) : scan_launch<T, Op, false, false>(s, a);
+  constexpr int KA = ScanTune<T>::K, KU = ScanTune<T>::K_UNALIGNED;
+  if (reverse)
+    return aligned ? scan_launch<T, Op, true, true, KA>(s, a) : scan_launch<T, Op, true, false, KU>(s, a);
+  if (small) return scan_launch<T, Op, false, true, kSmallK>(s, a);
+  return aligned ? scan_launch<T, Op, false, true, KA>(s, a) : scan_launch<T, Op, false, false, KU>(s, a);
 }
 
 // rank >= 2 scans: rows shorter than 2^20 elements in one fused launch

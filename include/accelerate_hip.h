@@ -223,11 +223,11 @@ int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const void* seed, co
 /* Allocation-free foldSeg / fold1Seg.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
- * semantics and kernels as acc_hip_fold_seg, but the skeleton workspace is a
- * fixed 16 MiB device array inside the library (one per device, held until
- * the process ends) instead of the stream's allocated workspace, so the call
- * allocates and frees nothing.  A call whose workspace would exceed 16 MiB
- * (about 2^28.9 Float32 or 2^27.9 Float64 elements per row) is refused with
+ * semantics and kernels as acc_hip_fold_seg, but the skeleton workspace and
+ * scratch are a fixed 16 MiB device array inside the library (one per device,
+ * held until the process ends) instead of the stream's allocations, so the
+ * call allocates and frees nothing.  A call whose records would not fit
+ * (about 2^28.8 Float32 or 2^27.8 Float64 elements per row) is refused with
  * ACC_ERROR_INVALID_VALUE before anything is launched.  All calls on one
  * device share that device's array: calls that overlap on one device (other
  * streams, other threads) race and must be serialised by the caller. */

@@ -280,6 +280,8 @@ SIGNATURES = {
     "acc_hip_server_stats": (c_int, [c_void_p, POINTER(ServerStats)]),
     "acc_hip_server_reset_stats": (c_int, [c_void_p]),
     "acc_hip_server_destroy": (c_int, [c_void_p]),
+    "acc_hip_bandwidth_probe": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t, c_int,
+                                        POINTER(ctypes.c_double)]),
 }
 
 
@@ -727,6 +729,19 @@ class Server:
             self.destroy()
         except Exception:
             pass
+
+
+PROBE_READ, PROBE_COPY, PROBE_WRITE = 0, 1, 2
+
+
+def bandwidth_probe(stream, kind, vectors, blocks_per_cu, nontemporal, src, dst, nbytes, reps=10):
+    """Mean microseconds per launch of the read / copy / write probe kernel."""
+    us = ctypes.c_double()
+    check(lib().acc_hip_bandwidth_probe(stream.handle, kind, vectors, blocks_per_cu, 1 if nontemporal else 0,
+                                        c_void_p(src.ptr) if src is not None else None,
+                                        c_void_p(dst.ptr) if dst is not None else None, nbytes, reps,
+                                        ctypes.byref(us)))
+    return us.value
 
 
 def copy_to_host(stream, ptr, count, dtype):

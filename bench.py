@@ -58,6 +58,21 @@ def timed(stream, fn, reps=20, warmup=2, stat="median"):
     return float(np.mean(t) if stat == "mean" else np.median(t))
 
 
+def timed_run(stream, fn, reps=50, warmup=3):
+    """Mean milliseconds per call of `reps` back-to-back calls of fn() between
+    one pair of HIP events on `stream` (no event between the calls)."""
+    for _ in range(warmup):
+        fn()
+    e0, e1 = A.Event(stream.device), A.Event(stream.device)
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
 def seg_bytes(n, m, b=4, ib=8):
     return n * b + (m + 1) * ib + m * b
 
@@ -161,6 +176,37 @@ def bench_batched(s, args):
     g.destroy()
     b.destroy()
     return out
+
+
+def bench_ceilings(s):
+    """The device's HBM ceilings: read-only, copy and write-only probe kernels
+    over 2 GiB (1 GiB each way for the copy) for vectors in flight per lane x
+    workgroups per CU x non-temporal hint; the best of each kind."""
+    nbytes = 1 << 31
+    src, dst = A.DeviceArray(nbytes, np.uint8), A.DeviceArray(nbytes, np.uint8)
+    A.check(A.lib().acc_hip_fill_async(s.handle, A.c_void_p(src.ptr), np.ones(1, np.uint32).ctypes.data_as(A.c_void_p),
+                                       4, nbytes // 4))
+    table = []
+    for kind, name in ((A.PROBE_READ, "read"), (A.PROBE_COPY, "copy"), (A.PROBE_WRITE, "write")):
+        nb = nbytes // 2 if kind == A.PROBE_COPY else nbytes
+        moved = 2 * nb if kind == A.PROBE_COPY else nb
+        for vec in (4, 8, 16):
+            for bpc in (1, 2, 4, 8):
+                for nt in (False, True):
+                    us = A.bandwidth_probe(s, kind, vec, bpc, nt, src if kind != A.PROBE_WRITE else None,
+                                           dst if kind != A.PROBE_READ else None, nb, reps=10)
+                    table.append({"kind": name, "vectors": vec, "blocks_per_cu": bpc, "nontemporal": nt,
+                                  "us": round(us, 1), "gbps": round(moved / (us * 1e-6) / 1e9, 1)})
+    best = {}
+    for r in table:
+        if r["kind"] not in best or r["gbps"] > best[r["kind"]]["gbps"]:
+            best[r["kind"]] = r
+    for k, r in best.items():
+        log(f"ceiling {k}: {r['gbps']:.0f} GB/s ({100 * r['gbps'] / HBM_PEAK_GBS:.1f}% of peak) with "
+            f"{r['vectors']} vectors/lane, {r['blocks_per_cu']} workgroups/CU, non-temporal {r['nontemporal']}")
+    src.free()
+    dst.free()
+    return {"best": best, "table": table}
 
 
 def bench_serving(s, args):
@@ -392,6 +438,7 @@ def main():
     ap.add_argument("--serve-round-mib", type=int, default=1024, help="algorithmic MiB per fused round at most")
     ap.add_argument("--serve-vectors", type=int, default=8, help="vectors per thread of the fused launch (2, 4, 8)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
+    ap.add_argument("--no-ceilings", action="store_true", help="skip the HBM ceiling probes")
     ap.add_argument("--kernel-only", action="store_true",
                     help="(internal) run only the dominant kernel and a 1 GiB copy, for counter collection")
     args = ap.parse_args()
@@ -476,8 +523,8 @@ def main():
     # ---------------------------------------------- dominant kernel roofline
     k_bytes = seg_bytes(N_VALUES, N_SEG)
     k_fn = lambda: A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)
-    k_ms = timed(s, k_fn, 50, stat="mean")
-    k_med = timed(s, k_fn, 20)
+    k_ms = timed_run(s, k_fn, 50)  # mean over 50 back-to-back calls
+    k_med = timed(s, k_fn, 20)     # median of 20 calls, each between its own two events
     k_gbs = k_bytes / (k_ms * 1e-3) / 1e9
     roofline = {
         "kernel": "fold1Seg (+) Float32 call: k_tile_seg + k_fold_seg + k_seg_fix",
@@ -578,6 +625,8 @@ def main():
         del d_vol, d_vout
         del d_out
 
+    # ------------------------------------------------------- HBM ceilings
+    ceilings = None if args.no_ceilings else bench_ceilings(s)
     # ------------------------------------------------------- batched (C4)
     batched = None if args.no_batched else bench_batched(s, args)
     # ------------------------------------------------------- serving (C5)
@@ -635,6 +684,7 @@ def main():
         "fp_error": fp_error,
         "cpu_baseline": cpu,
         "skeletons": skel,
+        "ceilings": ceilings,
         "batched": batched,
         "serving": serving,
         "device": {"name": props.name.decode(), "arch": props.arch.decode(), "cus": props.compute_units},

@@ -658,6 +658,20 @@ int acc_hip_server_stats(acc_server_t s, acc_server_stats* st);
 int acc_hip_server_reset_stats(acc_server_t s);
 int acc_hip_server_destroy(acc_server_t s);
 
+/* ================================================================ HBM ceilings
+ * No counterpart in the reference: it calibrates the roofline fractions the
+ * skeletons are measured against.  Times `reps` back-to-back launches of a
+ * read-only reduction (src), a copy (src -> dst) or a write-only fill (dst)
+ * over `bytes`, with `vectors` (4, 8, 16) 16-byte loads or stores in flight
+ * per lane, `blocks_per_cu` 256-thread workgroups per CU and, when
+ * `nontemporal`, the non-temporal hint on the loads (read, copy) or the
+ * stores (write); *us = mean microseconds per launch.  The kernels stride
+ * over whole grid-wide steps of vectors: a tail shorter than one step is left
+ * out (none when bytes is a multiple of 16 * 256 * vectors * workgroups). */
+typedef enum acc_probe_kind { ACC_PROBE_READ = 0, ACC_PROBE_COPY = 1, ACC_PROBE_WRITE = 2 } acc_probe_kind;
+int acc_hip_bandwidth_probe(acc_stream_t s, int kind, int vectors, int blocks_per_cu, int nontemporal,
+                            const void* src, void* dst, size_t bytes, int reps, double* us);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,3 +167,25 @@ def test_pinned_host_memory_round_trip(stream):
     stream.synchronize()
     assert np.array_equal(host, -np.arange(n, dtype=np.int32))
     A.check(L().acc_hip_host_free(p))
+
+
+def test_bandwidth_probe_moves_the_bytes_it_times(stream):
+    """The ceiling probe: the copy moves src into dst, the write-only fill
+    writes every 16-byte vector, the read leaves both untouched; times are
+    positive; bad shapes are refused."""
+    n = 1 << 22
+    src = A.DeviceArray.from_numpy(stream, np.arange(n, dtype=np.uint32))
+    dst = A.DeviceArray.from_numpy(stream, np.zeros(n, np.uint32))
+    for vec in (4, 8, 16):
+        for nt in (False, True):
+            assert A.bandwidth_probe(stream, A.PROBE_READ, vec, 2, nt, src, None, 4 * n, reps=2) > 0
+    assert np.array_equal(dst.to_numpy(stream), np.zeros(n, np.uint32))
+    assert A.bandwidth_probe(stream, A.PROBE_COPY, 4, 2, True, src, dst, 4 * n, reps=2) > 0
+    assert np.array_equal(dst.to_numpy(stream), np.arange(n, dtype=np.uint32))
+    assert A.bandwidth_probe(stream, A.PROBE_WRITE, 8, 1, False, None, dst, 4 * n, reps=2) > 0
+    got = dst.to_numpy(stream).reshape(-1, 4)
+    assert np.all(got[:, :3] == np.array([1, 2, 3], np.uint32))
+    with pytest.raises(A.AccError):
+        A.bandwidth_probe(stream, A.PROBE_READ, 5, 2, False, src, None, 4 * n)
+    with pytest.raises(A.AccError):
+        A.bandwidth_probe(stream, A.PROBE_COPY, 4, 2, False, src, None, 4 * n)

@@ -280,6 +280,7 @@ SIGNATURES = {
     "acc_hip_server_stats": (c_int, [c_void_p, POINTER(ServerStats)]),
     "acc_hip_server_reset_stats": (c_int, [c_void_p]),
     "acc_hip_server_destroy": (c_int, [c_void_p]),
+    "acc_hip_stream_native": (c_int, [c_void_p, POINTER(c_void_p)]),
     "acc_hip_bandwidth_probe": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t, c_int,
                                         POINTER(ctypes.c_double)]),
 }

@@ -178,6 +178,7 @@ static void build_node(const std::string& text, Node& nd) {
   if ((int)nd.free.size() > accjit::kMaxFree) throw Error("at most 8 free arrays");
   Emitter proto;
   proto.free_types = nd.free;
+  proto.foreign = std::make_shared<std::map<std::string, std::string>>();
 
   std::string src = "#include \"jit/prelude.hpp\"\n#include \"jit/skel.hpp\"\nusing namespace accjit;\n";
   auto fn_struct = [&](const char* name, const std::string& sig, const std::string& body) {
@@ -235,7 +236,12 @@ static void build_node(const std::string& text, Node& nd) {
     src += fn_struct("Bnd", Ti + " operator()(const " + Ti + "* in, int64_t y, int64_t x, int64_t H, int64_t W)", bbody);
     src += "extern \"C\" __global__ __launch_bounds__(256) void acc_jit_stencil(Args a) {\n  stencil2d<" + Ti + ", " + To +
            ", Fs, Bnd, " + std::to_string(nd.ry) + ", " + std::to_string(nd.rx) + ">(a, Fs{a.fv}, Bnd{a.fv});\n}\n";
-    nd.source = src;
+    // foreign scalar functions, at file scope ahead of the functors that call them
+  std::string defs;
+  for (const auto& kv : *proto.foreign) defs += kv.second;
+  const std::string head = "using namespace accjit;\n";
+  const size_t at = src.find(head) + head.size();
+  nd.source = src.substr(0, at) + defs + src.substr(at);
     nd.key = node_key(src);
     return;
   }
@@ -266,7 +272,12 @@ static void build_node(const std::string& text, Node& nd) {
     src += manifest_storer("St", "E", nd.elt);
     src += "extern \"C\" __global__ __launch_bounds__(256) void acc_jit_generate(Args a) {\n  generate<E>(a, Gen{a.fv}, St{" +
            ptr_list("a.out") + ", a.vec});\n}\n";
-    nd.source = src;
+    // foreign scalar functions, at file scope ahead of the functors that call them
+  std::string defs;
+  for (const auto& kv : *proto.foreign) defs += kv.second;
+  const std::string head = "using namespace accjit;\n";
+  const size_t at = src.find(head) + head.size();
+  nd.source = src.substr(0, at) + defs + src.substr(at);
     nd.key = node_key(src);
     return;
   }
@@ -375,7 +386,12 @@ static void build_node(const std::string& text, Node& nd) {
     src += std::string("extern \"C\" __global__ __launch_bounds__(256) void acc_jit_permute(Args a) {\n  permute<E, Fn, Ld, Proj, Out, ") +
            (nd.cas ? "true" : "false") + ">(a, " + FN + ", " + LD + ", Proj{a.fv}, Out{" + ptr_list("a.out") + "});\n}\n";
   }
-  nd.source = src;
+  // foreign scalar functions, at file scope ahead of the functors that call them
+  std::string defs;
+  for (const auto& kv : *proto.foreign) defs += kv.second;
+  const std::string head = "using namespace accjit;\n";
+  const size_t at = src.find(head) + head.size();
+  nd.source = src.substr(0, at) + defs + src.substr(at);
   nd.key = node_key(src);
 }
 

@@ -39,6 +39,11 @@ int sc_parse(const std::string& s) {
 [[noreturn]] static void err(const std::string& m) { throw Error(m); }
 
 std::string SX::str() const {
+  if (atom && quoted) {
+    std::string q = "\"";
+    for (char c : a) q += (c == '"' || c == '\\') ? std::string("\\") + c : std::string(1, c);
+    return q + "\"";
+  }
   if (atom) return a;
   std::string s = "(";
   for (size_t i = 0; i < l.size(); ++i) s += (i ? " " : "") + l[i].str();
@@ -63,6 +68,19 @@ SX parse_sx(const std::string& text) {
     if (c == '(') {
       stack.emplace_back();
       ++i;
+    } else if (c == '"') {
+      // a string literal: \" and \\ escape, anything else is kept as is
+      SX a;
+      a.atom = true;
+      a.quoted = true;
+      size_t j = i + 1;
+      for (; j < text.size() && text[j] != '"'; ++j) {
+        if (text[j] == '\\' && j + 1 < text.size() && (text[j + 1] == '"' || text[j + 1] == '\\')) ++j;
+        a.a += text[j];
+      }
+      if (j >= text.size()) err("unterminated string literal");
+      stack.back().l.push_back(std::move(a));
+      i = j + 1;
     } else if (c == ')') {
       if (stack.size() < 2) err("unbalanced ')'");
       SX done = std::move(stack.back());
@@ -243,6 +261,56 @@ Val Emitter::emit(const SX& e) {
     const std::string rd = std::string("ap::read<") + kStore[t] + ">(fv.p[" + std::to_string(k) + "], fv.n[" +
                            std::to_string(k) + "], " + i.v[0] + ")";
     return scalar(t, bind(t, t == BOOL ? "(" + rd + " != 0)" : rd));
+  }
+  if (op == "foreign") {
+    // (foreign "BODY" (fun (x ..) FALLBACK) E ..): a foreign scalar function
+    // (ForeignExp, PTX/Foreign.hs:74-84): BODY is the body of a device
+    // function of the parameters x .. (typed by the arguments E ..), compiled
+    // into the kernel; the fallback gives the function's meaning where no
+    // foreign code runs and, type-checked on the same arguments, its result
+    // type.  Numeric scalars only.
+    if (na < 2 || !e.l[1].atom || !e.l[1].quoted) err("foreign: expected (foreign \"body\" (fun (x ..) e) args ..)");
+    const SX& fb = e.l[2];
+    if (!fb.is("fun") || fb.l.size() != 3 || fb.l[1].atom) err("foreign: the fallback must be (fun (x ..) e)");
+    const std::vector<SX>& ps = fb.l[1].l;
+    if (ps.size() != na - 2) err("foreign: the fallback takes " + std::to_string(ps.size()) + " arguments, given " +
+                                 std::to_string(na - 2));
+    if (!foreign) err("foreign: not available here");
+    std::vector<Val> args;
+    std::vector<int> ts;
+    for (size_t k = 3; k < e.l.size(); ++k) {
+      args.push_back(emit(e.l[k]));
+      ts.push_back(want_scalar(args.back(), "foreign"));
+      if (!sc_num(ts.back())) err("foreign: numeric scalar arguments only");
+    }
+    // the fallback, typed on placeholders of the arguments' types (closed:
+    // only its parameters and the free arrays are in scope)
+    Emitter chk;
+    chk.free_types = free_types;
+    chk.foreign = foreign;
+    for (size_t k = 0; k < ps.size(); ++k) {
+      if (!ps[k].atom) err("foreign: parameter names must be symbols");
+      Val v;
+      v.t = {ts[k]};
+      v.v = {std::string("((") + kCType[ts[k]] + ")0)"};
+      chk.env.emplace_back(ps[k].a, v);
+    }
+    const Val fr = chk.emit(fb.l[2]);
+    const int rt = want_scalar(fr, "foreign fallback");
+    if (!sc_num(rt)) err("foreign: numeric scalar result only");
+    std::string sig = std::string(kCType[rt]) + "(";
+    for (size_t k = 0; k < ts.size(); ++k) sig += std::string(k ? ", " : "") + kCType[ts[k]] + " " + ps[k].a;
+    sig += ")";
+    unsigned long long h = 0xcbf29ce484222325ull;
+    for (unsigned char c : sig + "\n" + e.l[1].a) { h ^= c; h *= 0x100000001b3ull; }
+    char name[40];
+    snprintf(name, sizeof name, "acc_foreign_%016llx", h);
+    (*foreign)[name] = std::string("static __device__ __forceinline__ ") + kCType[rt] + " " + name +
+                       sig.substr(sig.find('(')) + " {\n" + e.l[1].a + "\n}\n";
+    std::string call = std::string(name) + "(";
+    for (size_t k = 0; k < args.size(); ++k) call += (k ? ", " : "") + args[k].v[0];
+    call += ")";
+    return scalar(rt, bind(rt, call));
   }
   if (op == "nb") {
     arity(2);

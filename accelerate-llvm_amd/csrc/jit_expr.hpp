@@ -6,6 +6,8 @@
 // device code calling the semantics of jit/prelude.hpp.
 #pragma once
 #include <stdint.h>
+#include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,6 +36,7 @@ struct Error : std::runtime_error {
 // S-expressions
 struct SX {
   bool atom = false;
+  bool quoted = false;  // a string literal ("..."): `a` holds its text
   std::string a;
   std::vector<SX> l;
   bool is(const char* head) const { return !atom && !l.empty() && l[0].atom && l[0].a == head; }
@@ -56,6 +59,9 @@ struct Emitter {
   // stencil functions: (nb dy dx) reads the neighbourhood around `nb_ptr`
   bool stencil = false;
   int nb_ry = 0, nb_rx = 0, nb_stride = 0, nb_type = F32;
+  // foreign scalar functions met so far: name -> file-scope definition
+  // (shared by every copy of the prototype emitter of one node)
+  std::shared_ptr<std::map<std::string, std::string>> foreign;
 
   Val emit(const SX& e);
   std::string bind(int t, const std::string& expr);

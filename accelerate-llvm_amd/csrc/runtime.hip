@@ -481,6 +481,12 @@ int acc_hip_stream_device(acc_stream_t s, int* device) {
   return ACC_SUCCESS;
 }
 
+int acc_hip_stream_native(acc_stream_t s, void** hip_stream) {
+  if (!s || !hip_stream) return fail(ACC_ERROR_INVALID_VALUE, "null argument");
+  *hip_stream = (void*)s->stream;
+  return ACC_SUCCESS;
+}
+
 int acc_hip_stream_reserve(acc_stream_t s, size_t status_bytes) {
   SyncCtl* c;
   uint64_t* st;

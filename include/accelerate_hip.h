@@ -155,6 +155,12 @@ int acc_hip_stream_destroy(acc_stream_t s);
 int acc_hip_stream_synchronize(acc_stream_t s);
 int acc_hip_stream_query(acc_stream_t s); /* ACC_SUCCESS = idle, ACC_ERROR_NOT_READY = busy */
 int acc_hip_stream_device(acc_stream_t s, int* device);
+/* The HIP stream under `s` (a hipStream_t), for foreign array functions
+ * (ForeignAcc, PTX/Foreign.hs:74-84: a host function given the backend's
+ * stream and its arrays' device pointers): work a foreign function enqueues
+ * there is ordered with the skeletons on `s`, and an event recorded on `s`
+ * after it is the foreign result's future. */
+int acc_hip_stream_native(acc_stream_t s, void** hip_stream);
 /* Pre-size the stream's skeleton workspace (bytes of tile-status words, and as
  * many bytes of untagged per-tile scratch), so that no allocation happens
  * later, e.g. while the stream is being captured. */
@@ -376,6 +382,12 @@ int acc_hip_transform(acc_stream_t s, acc_type t, acc_ixmap p, acc_unop f, const
  *   x | (const T lit) | (let ((x E) ..) E) | (cond E E E) | (tuple E ..) | (prj k E)
  *   | (read k E)   element E (an i64) of free array k, clamped into the array
  *   | (nb dy dx)   stencil functions: the neighbour at offset (dy, dx)
+ *   | (foreign "BODY" (fun (x ..) E) E ..)   a foreign scalar function
+ *                  (ForeignExp, PTX/Foreign.hs:74-84): BODY is the C++ body of
+ *                  a device function of the parameters x .. (typed by the
+ *                  arguments), compiled into the kernel; the fallback (fun ..)
+ *                  is its meaning where no foreign code runs and gives its
+ *                  result type (numeric scalars; \" and \\ escape in BODY)
  *   | (op E ..)    op one of the PrimFun primitives (LLVM/CodeGen/Exp.hs:244-310):
  *       add sub mul neg abs signum quot rem quotrem div mod divmod band bor bxor
  *       bnot shiftl shiftr rotatel rotater popcount clz ctz fdiv recip sin cos tan

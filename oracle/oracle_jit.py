@@ -50,12 +50,26 @@ def storage(t):
 
 
 # ------------------------------------------------------------------ parser
+class Quoted(str):
+    """A string literal of a node (the body of a foreign function)."""
+
+
 def parse(text):
     toks, i = [], 0
     while i < len(text):
         c = text[i]
         if c.isspace():
             i += 1
+        elif c == '"':
+            j, lit = i + 1, []
+            while j < len(text) and text[j] != '"':
+                if text[j] == "\\" and j + 1 < len(text) and text[j + 1] in '"\\':
+                    j += 1
+                lit.append(text[j])
+                j += 1
+            assert j < len(text), "unterminated string literal"
+            toks.append(Quoted("".join(lit)))
+            i = j + 1
         elif c == ";":
             while i < len(text) and text[i] != "\n":
                 i += 1
@@ -74,7 +88,7 @@ def parse(text):
         nonlocal pos
         t = toks[pos]
         pos += 1
-        if t == "(":
+        if t == "(" and not isinstance(t, Quoted):
             out = []
             while toks[pos] != ")":
                 out.append(read())
@@ -203,6 +217,13 @@ class Interp:
             v = arr[np.clip(i, 0, len(arr) - 1)] if len(arr) else np.zeros(np.shape(i), arr.dtype)
             k = int(e[1])
             return [v.astype(np.bool_) if k < len(self.free_bool) and self.free_bool[k] else v]
+        if op == "foreign":
+            # a foreign scalar function: its meaning is the fallback, applied
+            # to the arguments in a closed environment (the body is device
+            # code the oracle does not run)
+            fun = e[2]
+            args = [self.ev(x, env) for x in e[3:]]
+            return self.ev(fun[2], dict(zip(fun[1], args)))
         if op == "nb":
             return [self.nb[(int(e[1]), int(e[2]))]]
         conv = op in ("fromintegral", "tofloating", "bitcast", "truncate", "round", "floor", "ceiling")

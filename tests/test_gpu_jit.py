@@ -485,3 +485,58 @@ def test_code_object_cache_and_embedded_images(stream, tmp_path, monkeypatch):
     assert not empty.exists() or not list(empty.glob("*.co"))  # nothing compiled or stored
     with pytest.raises(A.AccError, match="another node"):
         A.Kernel("(fold (elt i64) (fun (a b) (add a b)))", image=image)
+
+
+# ------------------------------------------------------------- foreign functions
+# (foreign "body" (fun (x ..) fallback) args ..): ForeignExp (PTX/Foreign.hs:74-84).
+# The device runs the body; the oracle evaluates the fallback; they must agree.
+BITREV32 = ("(let ((x1 (bor (band (shiftr x (const i64 1)) (const u32 0x55555555)) "
+            "(shiftl (band x (const u32 0x55555555)) (const i64 1)))) "
+            "(x2 (bor (band (shiftr x1 (const i64 2)) (const u32 0x33333333)) "
+            "(shiftl (band x1 (const u32 0x33333333)) (const i64 2)))) "
+            "(x3 (bor (band (shiftr x2 (const i64 4)) (const u32 0x0f0f0f0f)) "
+            "(shiftl (band x2 (const u32 0x0f0f0f0f)) (const i64 4)))) "
+            "(x4 (bor (band (shiftr x3 (const i64 8)) (const u32 0x00ff00ff)) "
+            "(shiftl (band x3 (const u32 0x00ff00ff)) (const i64 8))))) "
+            "(bor (shiftr x4 (const i64 16)) (shiftl x4 (const i64 16))))")
+
+
+def test_foreign_scalar_function_in_a_generator(stream, kernels):
+    """A device intrinsic outside the primitive set (bit reversal) as a foreign
+    function, with a fallback built from shifts and masks."""
+    node = ("(generate (elt u32 i64) (fun (i) (tuple "
+            "(foreign \"return __builtin_bitreverse32(x);\" (fun (x) " + BITREV32 + ") (fromintegral u32 i)) "
+            "(foreign \"return x * x + 7;\" (fun (x) (add (mul x x) (const i64 7))) i))))")
+    assert "__builtin_bitreverse32" in A.node_source(node)
+    k = kernels(node)
+    n = (1 << 20) + 77
+    out = [A.DeviceArray(n, np.uint32), A.DeviceArray(n, np.int64)]
+    k.run(stream, None, out, 1, n)
+    got, want = host(stream, out), J.generate(node, 1, n)
+    assert same(got, want)
+    assert got[0][1] == 1 << 31 and got[0][3] == 3 << 30  # reversal of 1 and 3
+
+
+def test_foreign_function_in_a_fused_fold_input(stream, kernels):
+    """The same foreign function inside a delayed producer fused into a fold,
+    with a second foreign function of two arguments as the combine's operand."""
+    node = ("(fold (elt u32) (fun (a b) (add a b)) (seed (const u32 3)) (input (fun (i) "
+            "(foreign \"return (x < y) ? y : x;\" (fun (x y) (max x y)) "
+            "(foreign \"return __builtin_bitreverse32(x);\" (fun (x) " + BITREV32 + ") (fromintegral u32 i)) "
+            "(read 0 i)))) (free u32))")
+    k = kernels(node)
+    n = (1 << 22) + 5
+    f = np.random.default_rng(4).integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+    df = A.DeviceArray.from_numpy(stream, f)
+    out = A.DeviceArray(1, np.uint32)
+    k.run(stream, None, [out], 1, n, free=[df])
+    assert out.to_numpy(stream)[0] == J.fold(node, [], 1, n, free=[f])[0][0]
+
+
+def test_foreign_function_errors():
+    with pytest.raises(A.AccError, match="foreign"):  # fallback arity
+        A.node_source("(generate (elt i64) (fun (i) (foreign \"return x;\" (fun (x y) x) i)))")
+    with pytest.raises(A.AccError, match="foreign"):  # not a string body
+        A.node_source("(generate (elt i64) (fun (i) (foreign body (fun (x) x) i)))")
+    with pytest.raises(A.AccError, match="foreign"):  # Bool
+        A.node_source("(generate (elt bool) (fun (i) (foreign \"return x;\" (fun (x) x) (lt i (const i64 3)))))")

@@ -189,3 +189,41 @@ def test_bandwidth_probe_moves_the_bytes_it_times(stream):
         A.bandwidth_probe(stream, A.PROBE_READ, 5, 2, False, src, None, 4 * n)
     with pytest.raises(A.AccError):
         A.bandwidth_probe(stream, A.PROBE_COPY, 4, 2, False, src, None, 4 * n)
+
+
+def test_foreign_array_function_between_skeletons(stream):
+    """aforeign (ForeignAcc, PTX/Foreign.hs:74-84): a foreign array function
+    runs on the backend's stream between two skeletons -- here the stand-alone
+    axpy kernel launched by raw HIP on the stream acc_hip_stream_native gives
+    (the way a foreign function calls a library), then a fold -- and an event
+    recorded after it is the result's future; the result equals the fallback
+    computed by the skeletons alone."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipModuleLaunchKernel.restype = ctypes.c_int
+    native = ctypes.c_void_p()
+    A.check(L().acc_hip_stream_native(stream.handle, ctypes.byref(native)))
+    assert native.value
+    m, keep = load_module()
+    f = get_function(m, "acc_foreign_axpy")
+    n = (1 << 20) + 3
+    x = np.random.default_rng(5).integers(-8, 8, size=n).astype(np.float32)
+    dx = A.DeviceArray.from_numpy(stream, x)
+    dy = A.DeviceArray(n, np.float32)
+    A.scan(stream, A.ADD, A.LEFT_TO_RIGHT, None, dx, 1, n, dy)  # skeleton: y = scanl1 (+) x
+    a, px, py, cn = ctypes.c_float(3.0), ctypes.c_void_p(dx.ptr), ctypes.c_void_p(dy.ptr), ctypes.c_int64(n)
+    args = (ctypes.c_void_p * 4)(*[ctypes.cast(ctypes.byref(v), ctypes.c_void_p) for v in (a, px, py, cn)])
+    # the foreign function: y = 3 x + y, by raw HIP on the backend's stream
+    assert hip.hipModuleLaunchKernel(f, 1024, 1, 1, 256, 1, 1, 0, native, args, None) == 0
+    out = A.DeviceArray(1, np.float32)
+    A.fold(stream, A.ADD, np.float32(0), dy, 1, n, out)  # skeleton on the foreign result
+    ev = A.Event(0)
+    ev.record(stream)  # the future of the whole pipeline
+    ev.synchronize()
+    # the fallback's value: every scan prefix and every 3 x + y is an exact
+    # small integer in Float32, so the device's y equals this one; the fold
+    # is held to the FP32 bound
+    y = np.cumsum(x, dtype=np.float64)
+    assert np.array_equal(dy.to_numpy(stream).astype(np.float64), 3.0 * x + y)
+    v = 3.0 * x.astype(np.float64) + y
+    assert abs(float(out.to_numpy(stream)[0]) - v.sum()) <= 1e-5 * np.log2(n) * np.abs(v).sum()
+    A.check(L().acc_hip_module_unload(m))

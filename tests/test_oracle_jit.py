@@ -357,3 +357,30 @@ def test_bad_nodes_are_rejected_with_a_message(bad, msg):
     with pytest.raises(A.AccError) as e:
         A.node_source(bad)
     assert msg in str(e.value) and e.value.status == A.INVALID_VALUE
+
+
+def test_foreign_is_its_fallback_and_strings_parse():
+    """(foreign "body" (fun (x ..) e) args ..) means the fallback applied to
+    the arguments, in a closed environment; string literals keep escaped
+    quotes and backslashes."""
+    e = J.parse('(foreign "return x; // \\"q\\" \\\\n" (fun (x) x) i)')
+    assert e[1] == 'return x; // "q" \\n'
+    i = np.arange(-5, 6, dtype=np.int64)
+    got = J.evaluate('(foreign "return x * x + 7;" (fun (x) (add (mul x x) (const i64 7))) i)', {"i": i})[0]
+    assert np.array_equal(got, i * i + 7)
+    # the fallback sees only its parameters: i inside it is the argument, not the outer i
+    got = J.evaluate('(foreign "" (fun (i) (mul i (const i64 2))) (add i (const i64 1)))', {"i": i})[0]
+    assert np.array_equal(got, (i + 1) * 2)
+    # bit reversal by shifts and masks against Python
+    x = np.array([1, 3, 0x80000000, 0x12345678, 0xffffffff], np.uint32)
+    masks = ["0x55555555", "0x33333333", "0x0f0f0f0f", "0x00ff00ff"]
+    body, v = "", "x"
+    for k, m in enumerate(masks):
+        s = 1 << k
+        body += (f"(x{k + 1} (bor (band (shiftr {v} (const i64 {s})) (const u32 {m})) "
+                 f"(shiftl (band {v} (const u32 {m})) (const i64 {s})))) ")
+        v = f"x{k + 1}"
+    rev = f"(let ({body}) (bor (shiftr x4 (const i64 16)) (shiftl x4 (const i64 16))))"
+    got = J.evaluate(f'(foreign "" (fun (x) {rev}) y)', {"y": x})[0]
+    want = np.array([int(f"{int(v):032b}"[::-1], 2) for v in x], np.uint32)
+    assert np.array_equal(got, want)

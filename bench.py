@@ -653,9 +653,16 @@ def main():
             t_cpu += time.perf_counter() - c0
             reps += 1
         cpu_gbs = step_bytes * reps / t_cpu / 1e9
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         cpu = {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": cores, "kind": "port",
                "sample": f"the full step ({reps} repetitions, {t_cpu:.1f} s) with the native backend's schedule "
-                         f"(divideWork over segments, {cores} threads)"}
+                         f"(divideWork over segments, {cores} threads)",
+               "cpu": model, "build": O.build_info()}
         log(f"CPU baseline: {cpu_gbs:.2f} GB/s on {cores} threads")
 
     result = {

@@ -479,6 +479,13 @@ void or_native_stencil2d_f32(int threads, int ry, int rx, const float* w, int bo
   free(r);
 }
 
+/* ------------------------------------------------------- build */
+#ifndef OR_MARCH
+#define OR_MARCH "default"
+#endif
+/* compiler and target of this build (reported beside the CPU baseline) */
+const char* or_build_info(void) { return "compiler " __VERSION__ ", -O3 -march=" OR_MARCH; }
+
 /* ------------------------------------------------------- generators */
 typedef struct { uint64_t s[4]; } xo;
 static uint64_t splitmix64(uint64_t* x) {

@@ -70,6 +70,7 @@ void or_native_stencil2d_f32(int threads, int ry, int rx, const float* w, int bo
                              const float* in, int64_t h, int64_t wd, float* out);
 
 /* Seeded generators (splitmix64 -> xoshiro256**), shared with bench/tests. */
+const char* or_build_info(void);
 void or_gen_uniform_i32(uint64_t seed, int32_t lo, int32_t hi, int32_t* out, int64_t n);
 void or_gen_uniform_f32(uint64_t seed, float lo, float hi, float* out, int64_t n);
 void or_gen_normal_f32(uint64_t seed, float* out, int64_t n);

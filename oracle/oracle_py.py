@@ -40,7 +40,13 @@ def lib():
         for name in ("or_native_fold_add_i32",):
             getattr(_lib, name).restype = c_int32
         _lib.or_divide_work1.restype = c_int64
+        _lib.or_build_info.restype = ctypes.c_char_p
     return _lib
+
+
+def build_info() -> str:
+    """Compiler and -march of the loaded oracle library."""
+    return lib().or_build_info().decode()
 
 
 def _ptr(a):

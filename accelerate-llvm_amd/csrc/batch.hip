@@ -835,6 +835,12 @@ static int batch_layout(acc_batch_t b, int block, int k, int R) {
     ACC_HIP_CHECK(hipMemcpy(b->d_map, b->plan.run_prob.data(), b->plan.run_prob.size() * sizeof(uint32_t),
                             hipMemcpyHostToDevice));
   ACC_HIP_CHECK(hipMemset(b->d_ws, 0, b->ws_bytes));
+  // The uploads and the zeroing above go to the null stream, which does not
+  // order the launches that follow on non-blocking streams (the tuner's own,
+  // every library stream): without this wait a zeroing still in flight can
+  // land inside the first launch, reset its ticket counter and erase status
+  // words already published, and a look-back then waits for ever.
+  ACC_HIP_CHECK(hipStreamSynchronize(nullptr));
   // warm the occupancy cache so a later launch inside a stream capture only
   // enqueues the kernel
   blocks_per_cu(shape_kernel(block, k), block);
@@ -958,6 +964,14 @@ int acc_hip_batch_create(int device, const acc_batch_problem* problems, int64_t 
   b->problems.assign(problems, problems + count);
   b->weights.resize(count);
   int rc = ACC_SUCCESS;
+  // the stencil weights are read here: wait for whatever is still producing
+  // them on other streams (the null stream's copy does not order after those)
+  bool stencils = false;
+  for (int64_t i = 0; i < count; ++i) stencils |= problems[i].kind == ACC_BATCH_STENCIL;
+  if (stencils) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = hip_fail(e, "batch: synchronising before reading stencil weights");
+  }
   for (int64_t i = 0; i < count && !rc; ++i) {
     const acc_batch_problem& p = problems[i];
     if (p.kind != ACC_BATCH_STENCIL) continue;

@@ -489,6 +489,9 @@ int acc_hip_server_create(int device, int slots, size_t hbm_budget, acc_server_t
     ACC_CHECK(grow_device(device, &rb.d, &rb.d_cap, (size_t)1 << 20));
     ACC_CHECK(grow_device(device, &rb.ws, &rb.ws_cap, (size_t)1 << 20));
     ACC_HIP_CHECK(hipMemset(rb.ws, 0, rb.ws_cap));
+    // (the zeroing runs on the null stream, which does not order the slot's
+    // non-blocking stream: it completes before any round can launch)
+    ACC_HIP_CHECK(hipStreamSynchronize(nullptr));
     hipEvent_t e;
     ACC_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     s->slot_event.push_back(e);

@@ -123,6 +123,20 @@ def test_batch_matches_oracle(stream, tile):
         check(stream, p, p["out"].to_numpy(stream))
 
 
+def test_batch_launch_right_after_create(stream):
+    """Creating a batch zeroes its workspace (tickets, status words); the first
+    launch follows at once on a non-blocking stream and must see that zeroing
+    complete.  A batch large enough that the zeroing is not instantaneous."""
+    probs = make_problems(stream, 1200, seed=29, max_log2=18)
+    for tile in (8, 4):
+        b = A.Batch(0, batch_of(probs), tile=tile)
+        b.launch(stream)
+        stream.synchronize()
+        for p in probs:
+            check(stream, p, p["out"].to_numpy(stream))
+        b.destroy()
+
+
 def test_batch_same_alone_and_in_mix(stream):
     """Every problem's batched result against the same problem through the plain
     C calls (acc_hip_batch_launch_plain): bit-exact where the arithmetic is."""

@@ -231,6 +231,10 @@ SIGNATURES = {
                                  c_int64, c_void_p]),
     "acc_hip_fold_seg_noalloc": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_int,
                                          c_void_p, c_int64, c_void_p]),
+    "acc_hip_seg_plan_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "acc_hip_seg_plan_build": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_int64]),
+    "acc_hip_fold_seg_planned": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "acc_hip_seg_plan_destroy": (c_int, [c_void_p]),
     "acc_hip_scan": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     "acc_hip_scan_prime": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
                                    c_void_p]),
@@ -471,6 +475,32 @@ def fold_seg_noalloc(stream, op, seed, src, outer, inner, offsets, nseg, out):
     keep, sp = _scalar(src.dtype, seed)
     check(lib().acc_hip_fold_seg_noalloc(stream.handle, acc_type(src.dtype), op, sp, _p(src), outer, inner,
                                          acc_type(offsets.dtype), _p(offsets), nseg, _p(out)))
+
+
+class SegPlan:
+    """A segment descriptor's tile plan (acc_hip_seg_plan_*): build it once for
+    an offsets array, then fold any number of arrays over those segments."""
+
+    def __init__(self, device=0):
+        h = c_void_p()
+        check(lib().acc_hip_seg_plan_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def build(self, stream, dtype, src, n, offsets, nseg):
+        """Plan nseg segments (offsets: DeviceArray/View) over rows of n elements
+        of dtype whose first row starts at src."""
+        check(lib().acc_hip_seg_plan_build(stream.handle, self.handle, acc_type(dtype), _p(src), n,
+                                           acc_type(offsets.dtype), _p(offsets), nseg))
+
+    def fold_seg(self, stream, op, seed, src, outer, out):
+        keep, sp = _scalar(src.dtype, seed)
+        check(lib().acc_hip_fold_seg_planned(stream.handle, self.handle, acc_type(src.dtype), op, sp, _p(src), outer,
+                                             _p(out)))
+
+    def destroy(self):
+        if self.handle:
+            check(lib().acc_hip_seg_plan_destroy(self.handle))
+            self.handle = None
 
 
 def scan(stream, op, direction, seed, src, outer, inner, out):

@@ -166,6 +166,15 @@ __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, in
   }
 }
 
+// The seed into every empty segment (a call on a tile plan built ahead, whose
+// pre-pass did not see the output).
+template <typename T, typename I>
+__global__ __launch_bounds__(256) void k_fill_empty(const I* off, int64_t nseg, T* out, T seed) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nseg; g += stride)
+    if (off[g] == off[g + 1]) out[g] = seed;
+}
+
 // One workgroup per tile.  The tile is read with coalesced dwordx4 loads
 // (all issued first), then processed in NH halves: a half's vectors are
 // written to LDS in a transposed layout, and each thread reads back one
@@ -548,9 +557,49 @@ template <typename T> struct SegTune {
   static constexpr int CAP = ACC_SEG_CAP;  // starts staged in LDS (beyond: searched in HBM)
 };
 
+// k_tile_seg over offsets of 4 or 8 bytes (out / fill_empty: the seed into
+// empty segments as well).
+template <typename T>
+int launch_tile_plan(acc_stream_t s, const void* offsets, size_t offset_bytes, int64_t nseg, int64_t tile,
+                     int64_t ntiles, int64_t shift, int64_t* first, int32_t* slots, T* out, bool fill_empty, T seed) {
+  const unsigned pg = (unsigned)((ntiles + 1 + 3) / 4);  // a wave per tile boundary
+  if (offset_bytes == 4)
+    hipLaunchKernelGGL((k_tile_seg<T, int32_t>), dim3(pg), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
+                       tile, (uint32_t)ntiles, shift, first, slots, out, fill_empty ? 1u : 0u, seed);
+  else
+    hipLaunchKernelGGL((k_tile_seg<T, int64_t>), dim3(pg), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
+                       tile, (uint32_t)ntiles, shift, first, slots, out, fill_empty ? 1u : 0u, seed);
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
+
+template <typename T>
+int launch_fill_empty(acc_stream_t s, const void* offsets, size_t offset_bytes, int64_t nseg, T* out, T seed) {
+  const DeviceInfo* di;
+  ACC_CHECK(device_info(s->device, &di));
+  const int64_t blocks = std::min<int64_t>(ceil_div(nseg, (int64_t)256), (int64_t)di->compute_units * 4);
+  const unsigned g = (unsigned)std::max<int64_t>(1, blocks);
+  if (offset_bytes == 4)
+    hipLaunchKernelGGL((k_fill_empty<T, int32_t>), dim3(g), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
+                       out, seed);
+  else
+    hipLaunchKernelGGL((k_fill_empty<T, int64_t>), dim3(g), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
+                       out, seed);
+  ACC_HIP_CHECK(hipGetLastError());
+  return ACC_SUCCESS;
+}
+
+// A tile plan built ahead (acc_hip_seg_plan_build) for the same offsets,
+// element type, element count and alignment: k_tile_seg's results.
+struct TilePlan {
+  const int64_t* first;
+  const int32_t* slots;
+};
+
 template <typename T, typename Op, typename Starts>
 int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg, int64_t e_end,
-                    const T* seed, bool check_empty, const void* offsets, size_t offset_bytes) {
+                    const T* seed, bool check_empty, const void* offsets, size_t offset_bytes,
+                    const TilePlan* plan = nullptr) {
   constexpr int BLOCK = SegTune<T>::BLOCK, K = SegTune<T>::K, CAP = SegTune<T>::CAP;
   constexpr int64_t TILE = (int64_t)BLOCK * K * Vec<T>::N;
   if ((uintptr_t)in % sizeof(T)) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg: input not element-aligned");
@@ -575,8 +624,9 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
   const size_t cv_bytes = al256((size_t)ntiles * sizeof(T)), cf_bytes = al256((size_t)ntiles);
   const size_t cg_bytes = (size_t)ntiles * 8;
   const size_t carry_bytes = al256(2 * cv_bytes + cf_bytes + cg_bytes);
-  const size_t segs_bytes = offsets ? ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256 : 0;
-  const size_t slot_bytes = offsets ? (size_t)ntiles * SLOT * sizeof(int32_t) : 0;
+  const bool own_plan = offsets && !plan;
+  const size_t segs_bytes = own_plan ? ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256 : 0;
+  const size_t slot_bytes = own_plan ? (size_t)ntiles * SLOT * sizeof(int32_t) : 0;
   constexpr int FIX = 256;  // tiles per k_seg_fix workgroup
   const int64_t nfix = ceil_div(ntiles, (int64_t)FIX);
   const size_t fix_bytes = (size_t)nfix * Tagged<SegV<T>>::NW * 8;
@@ -588,19 +638,18 @@ int fold_seg_launch(acc_stream_t s, const T* in, T* out, Starts st, int64_t nseg
                              (int64_t*)(cb + 2 * cv_bytes + cf_bytes)};
   const bool fill_empty = seed && check_empty;
   if constexpr (!std::is_same<Starts, UniformStarts>::value) {
-    int64_t* first = (int64_t*)((char*)ws + carry_bytes);
-    int32_t* slots = (int32_t*)((char*)ws + carry_bytes + segs_bytes);
-    const unsigned pg = (unsigned)((ntiles + 1 + 3) / 4);  // a wave per tile boundary
-    const T sd = seed ? *seed : T();
-    if (offset_bytes == 4)
-      hipLaunchKernelGGL((k_tile_seg<T, int32_t>), dim3(pg), dim3(256), 0, s->stream, (const int32_t*)offsets, nseg,
-                         TILE, (uint32_t)ntiles, shift, first, slots, out, fill_empty ? 1u : 0u, sd);
-    else
-      hipLaunchKernelGGL((k_tile_seg<T, int64_t>), dim3(pg), dim3(256), 0, s->stream, (const int64_t*)offsets, nseg,
-                         TILE, (uint32_t)ntiles, shift, first, slots, out, fill_empty ? 1u : 0u, sd);
-    ACC_HIP_CHECK(hipGetLastError());
-    st.first = first;
-    st.slots = slots;
+    if (plan) {
+      st.first = plan->first;
+      st.slots = plan->slots;
+      if (fill_empty) ACC_CHECK(launch_fill_empty(s, offsets, offset_bytes, nseg, out, *seed));
+    } else {
+      int64_t* first = (int64_t*)((char*)ws + carry_bytes);
+      int32_t* slots = (int32_t*)((char*)ws + carry_bytes + segs_bytes);
+      ACC_CHECK(launch_tile_plan<T>(s, offsets, offset_bytes, nseg, TILE, ntiles, shift, first, slots, out,
+                                    fill_empty, seed ? *seed : T()));
+      st.first = first;
+      st.slots = slots;
+    }
   }
   SegArgs<T, Starts> a{in0, out, st, nseg, shift, e_end + shift, (uint32_t)ntiles, seed ? 1u : 0u,
                        seed ? *seed : T(), carry};
@@ -660,6 +709,135 @@ extern "C" int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const voi
           rc = fold_seg_launch<T, Op, OffsetStarts<int64_t>>(
               s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr, nullptr, 0}, nseg, inner,
               (const T*)seed, true, offsets, 8);
+        if (rc) return rc;
+      }
+      return (int)ACC_SUCCESS;
+    });
+  });
+}
+
+// ------------------------------------------------------------ tile plans
+// A segment descriptor's tile plan, built once and used by every foldSeg over
+// the same offsets (one k_tile_seg for all of them).  The plan holds the
+// results of k_tile_seg for one element type, element count and alignment.
+struct acc_segplan_s {
+  int device = 0;
+  void* buf = nullptr;  // first[ntiles + 1] (256-byte aligned), then slots[ntiles * SLOT]
+  size_t cap = 0;
+  bool built = false;
+  acc_type type = ACC_INT32;
+  int64_t n = 0, nseg = 0, shift = 0, ntiles = 0;
+  acc_type offset_type = ACC_INT64;
+  const void* offsets = nullptr;
+};
+
+extern "C" int acc_hip_seg_plan_create(int device, acc_segplan_t* plan) {
+  if (!plan) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: null argument");
+  ACC_CHECK(set_device(device));
+  auto p = new acc_segplan_s();
+  p->device = device;
+  *plan = p;
+  return ACC_SUCCESS;
+}
+
+extern "C" int acc_hip_seg_plan_destroy(acc_segplan_t plan) {
+  if (!plan) return ACC_SUCCESS;
+  int rc = ACC_SUCCESS;
+  if (plan->buf) {
+    ACC_CHECK(set_device(plan->device));
+    const hipError_t e = hipDeviceSynchronize();  // folds still using it
+    if (e != hipSuccess) rc = hip_fail(e, "segment plan: synchronising before the free");
+    const int rf = dev_free(plan->device, plan->buf);
+    if (!rc) rc = rf;
+  }
+  delete plan;
+  return rc;
+}
+
+extern "C" int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in, int64_t n,
+                                      acc_type offset_type, const void* offsets, int64_t nseg) {
+  if (!s || !plan) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: null argument");
+  if (s->device != plan->device) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: stream on another device");
+  if (n < 0 || nseg < 0) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: negative extent");
+  if (offset_type != ACC_INT32 && offset_type != ACC_INT64)
+    return fail(ACC_ERROR_UNSUPPORTED, "segment plan: offsets must be Int32 or Int64");
+  const size_t B = type_size(t);
+  if (!B) return fail(ACC_ERROR_UNSUPPORTED, "segment plan: element type %d", (int)t);
+  if (nseg > 0 && !offsets) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: null offsets");
+  if ((uintptr_t)in % B) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: input not element-aligned");
+  ACC_CHECK(set_device(s->device));
+  plan->built = false;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    constexpr int64_t TILE = (int64_t)SegTune<T>::BLOCK * SegTune<T>::K * Vec<T>::N;
+    const int64_t shift = (int64_t)(((uintptr_t)in % 16) / sizeof(T));
+    const int64_t ntiles = n == 0 ? 0 : ceil_div(n + shift, TILE);
+    if (ntiles >= 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "segment plan: too many elements");
+    const size_t first_bytes = ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256;
+    const size_t bytes = first_bytes + (size_t)ntiles * SLOT * sizeof(int32_t);
+    if (bytes > plan->cap) {
+      if (plan->buf) {
+        // folds on any stream may still read the old plan
+        ACC_HIP_CHECK(hipDeviceSynchronize());
+        ACC_CHECK(dev_free(plan->device, plan->buf));
+        plan->buf = nullptr;
+        plan->cap = 0;
+      }
+      ACC_CHECK(dev_malloc(plan->device, bytes, &plan->buf));
+      plan->cap = bytes;
+    }
+    if (ntiles > 0 && nseg > 0)
+      ACC_CHECK(launch_tile_plan<T>(s, offsets, offset_type == ACC_INT32 ? 4 : 8, nseg, TILE, ntiles, shift,
+                                    (int64_t*)plan->buf, (int32_t*)((char*)plan->buf + first_bytes), nullptr,
+                                    false, T()));
+    plan->type = t;
+    plan->n = n;
+    plan->nseg = nseg;
+    plan->shift = shift;
+    plan->ntiles = ntiles;
+    plan->offset_type = offset_type;
+    plan->offsets = offsets;
+    plan->built = true;
+    return (int)ACC_SUCCESS;
+  });
+}
+
+extern "C" int acc_hip_fold_seg_planned(acc_stream_t s, acc_segplan_t plan, acc_type t, acc_op op, const void* seed,
+                                        const void* in, int64_t outer, void* out) {
+  if (!s || !plan) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): null argument");
+  if (!plan->built) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): the plan has not been built");
+  if (s->device != plan->device) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): stream on another device");
+  if (t != plan->type) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): element type differs from the plan's");
+  if (outer < 0) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): negative extent");
+  const int64_t inner = plan->n, nseg = plan->nseg;
+  if (outer == 0 || nseg == 0) return ACC_SUCCESS;
+  if (!in || !out) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): null array");
+  const size_t B = type_size(t);
+  if ((uintptr_t)in % B) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): input not element-aligned");
+  // every row must sit at the plan's position relative to a 16-byte boundary
+  if ((int64_t)(((uintptr_t)in % 16) / B) != plan->shift || (outer > 1 && (inner * (int64_t)B) % 16 != 0))
+    return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned): input alignment differs from the plan's");
+  ACC_CHECK(set_device(s->device));
+  const size_t I = plan->offset_type == ACC_INT32 ? 4 : 8;
+  const size_t first_bytes = ((size_t)(plan->ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256;
+  const TilePlan tp{(const int64_t*)plan->buf, (const int32_t*)((const char*)plan->buf + first_bytes)};
+  const void* offsets = plan->offsets;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    return with_op<T>(op, [&](auto f) {
+      using Op = decltype(f);
+      for (int64_t r = 0; r < outer; ++r) {
+        const T* rin = (const T*)in + r * inner;
+        T* rout = (T*)out + r * nseg;
+        int rc;
+        if (I == 4)
+          rc = fold_seg_launch<T, Op, OffsetStarts<int32_t>>(
+              s, rin, rout, OffsetStarts<int32_t>{(const int32_t*)offsets, nullptr, nullptr, 0}, nseg, inner,
+              (const T*)seed, true, offsets, 4, &tp);
+        else
+          rc = fold_seg_launch<T, Op, OffsetStarts<int64_t>>(
+              s, rin, rout, OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr, nullptr, 0}, nseg, inner,
+              (const T*)seed, true, offsets, 8, &tp);
         if (rc) return rc;
       }
       return (int)ACC_SUCCESS;

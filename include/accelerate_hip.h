@@ -226,6 +226,29 @@ int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const void* seed, co
                      int64_t outer, int64_t inner, acc_type offset_type, const void* offsets,
                      int64_t nseg, void* out);
 
+/* Segment tile plans: foldSeg over one offsets array, planned once.  No
+ * counterpart in the reference's FFI: its foldSeg maps segments to blocks
+ * again at every launch (PTX/Execute.hs:343-374).  Accelerate arrays are
+ * immutable, so a program that folds several arrays over one segment
+ * descriptor (the `scanl (+) 0` of the lengths shared by its foldSegs) can
+ * build the plan when the offsets are produced and keep it with them.
+ * acc_hip_seg_plan_build enqueues the plan's computation on `s` for `nseg`
+ * segments over rows of `n` elements of type `t` whose first row starts at
+ * `in` (the plan depends on the element type and on `in` modulo 16 bytes).
+ * acc_hip_fold_seg_planned is acc_hip_fold_seg over `outer` such rows, with
+ * the offsets, `n` and `nseg` of the plan; it refuses a type or an alignment
+ * other than the plan's (with outer > 1, rows must be 16-byte multiples).
+ * The offsets must not change while the plan is used, and a fold on another
+ * stream than the build's must be ordered after it (an event).  A build that
+ * needs a larger plan synchronises the device before freeing the old one. */
+typedef struct acc_segplan_s* acc_segplan_t;
+int acc_hip_seg_plan_create(int device, acc_segplan_t* plan);
+int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in, int64_t n,
+                           acc_type offset_type, const void* offsets, int64_t nseg);
+int acc_hip_fold_seg_planned(acc_stream_t s, acc_segplan_t plan, acc_type t, acc_op op, const void* seed,
+                             const void* in, int64_t outer, void* out);
+int acc_hip_seg_plan_destroy(acc_segplan_t plan);
+
 /* Allocation-free foldSeg / fold1Seg.  No counterpart in the reference: it is
  * for callers that cannot allocate device memory (code under graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,

@@ -1,0 +1,154 @@
+"""Segment tile plans: foldSeg over one offsets array with the plan built once
+(acc_hip_seg_plan_build + acc_hip_fold_seg_planned).  The planned call runs the
+same tile kernels on the same tiles as acc_hip_fold_seg, so its results must
+be bit-identical to the unplanned call's (floating sums included), which the
+oracle checks in turn (integers and max bit-exact, floating (+) within the FP32
+bound of the FP64 truth)."""
+import numpy as np
+import pytest
+
+import acc_hip as A
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+TYPES = [np.float32, np.int32, np.float64, np.int64]
+
+
+def segments(rng, n, nseg, empties=False):
+    if empties:
+        lens = rng.multinomial(n, np.ones(nseg) / nseg)
+    else:
+        cut = np.sort(rng.choice(np.arange(1, n), size=nseg - 1, replace=False))
+        lens = np.diff(np.concatenate([[0], cut, [n]]))
+    return np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+
+
+def data(rng, dt, n):
+    if np.dtype(dt).kind == "f":
+        return rng.standard_normal(n).astype(dt)
+    return rng.integers(-1000, 1000, size=n).astype(dt)
+
+
+def unplanned(stream, op, seed, d, outer, inner, d_off, nseg, dt):
+    o = A.DeviceArray.from_numpy(stream, np.full(outer * nseg, 7, dt))
+    A.fold_seg(stream, op, seed, d, outer, inner, d_off, nseg, o)
+    return o.to_numpy(stream)
+
+
+def planned(stream, plan, op, seed, d, outer, nseg, dt):
+    o = A.DeviceArray.from_numpy(stream, np.full(outer * nseg, 7, dt))
+    plan.fold_seg(stream, op, seed, d, outer, o)
+    return o.to_numpy(stream)
+
+
+def against_oracle(op, seed, x, off, got, outer=1):
+    want = O.fold_seg(op, seed, x, off, outer)
+    if seed is None:  # fold1Seg leaves empty segments alone
+        empty = np.tile(np.diff(off) == 0, outer)
+        want = np.where(empty, got, want)
+    if np.dtype(x.dtype).kind != "f" or op == A.MAX:
+        assert np.array_equal(got, want)
+    else:
+        lens = np.tile(np.diff(off), outer)
+        ab = np.abs(O.fold_seg(A.ADD, None, np.abs(x.astype(np.float64)), off, outer))
+        bound = 1e-5 * np.maximum(1, np.log2(np.maximum(lens, 2))) * ab + 1e-300
+        assert np.all(np.abs(got.astype(np.float64) - want.astype(np.float64)) <= bound + np.abs(want) * 1e-6)
+
+
+@pytest.mark.parametrize("dt", TYPES)
+@pytest.mark.parametrize("offs", [np.int32, np.int64])
+def test_planned_equals_unplanned(stream, dt, offs):
+    rng = np.random.default_rng(TYPES.index(dt) * 2 + (offs == np.int64))
+    n, nseg = 1_000_003, 3001
+    x = data(rng, dt, n)
+    off = segments(rng, n, nseg).astype(offs)
+    d, d_off = A.DeviceArray.from_numpy(stream, x), A.DeviceArray.from_numpy(stream, off)
+    plan = A.SegPlan(0)
+    plan.build(stream, dt, d, n, d_off, nseg)
+    for op in (A.ADD, A.MAX):  # two folds on one plan
+        got = planned(stream, plan, op, None, d, 1, nseg, dt)
+        assert np.array_equal(got, unplanned(stream, op, None, d, 1, n, d_off, nseg, dt))
+        against_oracle(op, None, x, off.astype(np.int64), got)
+    plan.destroy()
+
+
+def test_one_plan_many_arrays_and_modes(stream):
+    """Short segments (more than 512 starts in a tile), long ones (a segment
+    over many tiles), empty ones with a seed, for two different arrays."""
+    rng = np.random.default_rng(5)
+    n = 1 << 21
+    lens = np.concatenate([rng.integers(0, 4, size=60000), [300000], rng.integers(1, 2000, size=900)])
+    lens[-1] += n - lens.sum()
+    assert lens[-1] >= 0
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    nseg = lens.size
+    d_off = A.DeviceArray.from_numpy(stream, off)
+    plan = A.SegPlan(0)
+    for k, dt in enumerate((np.float32, np.int32)):
+        x = data(rng, dt, n)
+        d = A.DeviceArray.from_numpy(stream, x)
+        plan.build(stream, dt, d, n, d_off, nseg)
+        for op, seed in ((A.ADD, dt(3)), (A.MAX, None), (A.ADD, None)):
+            got = planned(stream, plan, op, seed, d, 1, nseg, dt)
+            assert np.array_equal(got, unplanned(stream, op, seed, d, 1, n, d_off, nseg, dt)), (dt, op, seed)
+            against_oracle(op, seed, x, off, got)
+    plan.destroy()
+
+
+def test_misaligned_views_and_rows(stream):
+    rng = np.random.default_rng(9)
+    inner, nseg, outer = 40000, 37, 3  # rows of 160000 bytes: 16-byte multiples
+    x = rng.standard_normal(outer * inner + 5).astype(np.float32)
+    off = segments(rng, inner, nseg)
+    base = A.DeviceArray.from_numpy(stream, x)
+    d_off = A.DeviceArray.from_numpy(stream, off)
+    plan = A.SegPlan(0)
+    v = base.view(1, outer * inner)  # 4 bytes past a 16-byte boundary
+    plan.build(stream, np.float32, v, inner, d_off, nseg)
+    got = planned(stream, plan, A.ADD, None, v, outer, nseg, np.float32)
+    assert np.array_equal(got, unplanned(stream, A.ADD, None, v, outer, inner, d_off, nseg, np.float32))
+    against_oracle(A.ADD, None, x[1:1 + outer * inner], off, got, outer)
+    # another alignment, or rows that are not 16-byte multiples, are refused
+    o = A.DeviceArray(outer * nseg, np.float32)
+    with pytest.raises(A.AccError):
+        plan.fold_seg(stream, A.ADD, None, base.view(2, inner), 1, o)
+    odd = A.SegPlan(0)
+    odd.build(stream, np.float32, base, inner + 1, A.DeviceArray.from_numpy(stream, segments(rng, inner + 1, 5)), 5)
+    with pytest.raises(A.AccError):
+        odd.fold_seg(stream, A.ADD, None, base, 2, o)
+    # the element type is the plan's
+    with pytest.raises(A.AccError):
+        plan.fold_seg(stream, A.ADD, None, A.DeviceArray(inner, np.int32), 1, A.DeviceArray(nseg, np.int32))
+    odd.destroy()
+    plan.destroy()
+
+
+def test_unbuilt_plan_and_rebuild_larger(stream):
+    rng = np.random.default_rng(13)
+    plan = A.SegPlan(0)
+    with pytest.raises(A.AccError):
+        plan.fold_seg(stream, A.ADD, None, A.DeviceArray(16, np.float32), 1, A.DeviceArray(1, np.float32))
+    for n, nseg in ((5000, 7), (3_000_000, 2000), (123, 123)):  # the second grows the plan
+        x = data(rng, np.int64, n)
+        off = segments(rng, n, nseg) if nseg < n else np.arange(n + 1, dtype=np.int64)
+        d, d_off = A.DeviceArray.from_numpy(stream, x), A.DeviceArray.from_numpy(stream, off)
+        plan.build(stream, np.int64, d, n, d_off, nseg)
+        got = planned(stream, plan, A.ADD, np.int64(-4), d, 1, nseg, np.int64)
+        against_oracle(A.ADD, np.int64(-4), x, off, got)
+    plan.destroy()
+
+
+def test_c2_size_planned_equals_unplanned(stream):
+    """The benchmark step's shape: 2^26 Float32 in 2^16 segments, (+) and max."""
+    from acc_hip import workloads as W
+    lens = W.segment_lengths(42, 1 << 16, 16384, 1 << 26)
+    off = W.offsets(lens)
+    x = W.normal_f32(42, 1 << 26)
+    d, d_off = A.DeviceArray.from_numpy(stream, x), A.DeviceArray.from_numpy(stream, off)
+    plan = A.SegPlan(0)
+    plan.build(stream, np.float32, d, x.size, d_off, lens.size)
+    for op in (A.ADD, A.MAX):
+        got = planned(stream, plan, op, None, d, 1, lens.size, np.float32)
+        assert np.array_equal(got, unplanned(stream, op, None, d, 1, x.size, d_off, lens.size, np.float32))
+    plan.destroy()

@@ -4,11 +4,13 @@
 One step = one pass of the segmented-reduction workload over resident data:
   Int64 scanl (+) 0 of 2^16 segment lengths (the offsets),
   Float32 fold1Seg (+) and fold1Seg max over 2^26 N(0,1) values in those
-  2^16 segments (lengths log-uniform in [1, 16384], normalised to 2^26).
+  2^16 segments (lengths log-uniform in [1, 16384], normalised to 2^26),
+  both on one tile plan of the offsets built in the step.
 `value` is the step's algorithmic HBM bytes / step time, in GB/s.
 
-Also reported: the dominant kernel's roofline (fold1Seg (+), timed with HIP
-events on its stream), each skeleton at its target size (GB/s and fraction of
+Also reported: the dominant kernel's roofline (the fold1Seg (+) call on the
+plan, timed with HIP events on its stream; the plan build and the unplanned
+call beside it), each skeleton at its target size (GB/s and fraction of
 the 8 TB/s HBM peak), a device-to-device copy ceiling, and the CPU baseline
 (the oracle's restatement of the native backend's schedule on the host's
 cores, on a bounded sample).
@@ -356,8 +358,10 @@ def kernel_only():
     d_vals = A.DeviceArray.from_numpy(s, W.normal_f32(SEED, N_VALUES))
     d_off = A.DeviceArray.from_numpy(s, off)
     d_sum = A.DeviceArray(N_SEG, np.float32)
+    plan = A.SegPlan(0)
+    plan.build(s, np.float32, d_vals, N_VALUES, d_off, N_SEG)
     for _ in range(5):
-        A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)
+        plan.fold_seg(s, A.ADD, None, d_vals, 1, d_sum)
     src, dst = A.DeviceArray(1 << 28, np.int32), A.DeviceArray(1 << 28, np.int32)
     for _ in range(3):
         A.check(A.lib().acc_hip_memcpy_d2d_async(s.handle, A.c_void_p(dst.ptr), A.c_void_p(src.ptr), 1 << 30))
@@ -365,13 +369,13 @@ def kernel_only():
     return 0
 
 
-# the kernels of one fold1Seg call: per-tile segment ranges, the tiles, the
+# the kernels of one fold1Seg call on a built tile plan: the tiles, the
 # carries between tiles
-FOLD_SEG_KERNELS = ("k_tile_seg", "k_fold_seg", "k_seg_fix")
+FOLD_SEG_KERNELS = ("k_fold_seg", "k_seg_fix")
 
 
 def measure_traffic():
-    """HBM bytes per fold1Seg (+) call (its three kernels) from two rocprofv3
+    """HBM bytes per fold1Seg (+) call (its two kernels) from two rocprofv3
     counter runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled so
     that the 1 GiB device copy in the same run reads/writes exactly 1 GiB.
     None on failure."""
@@ -414,7 +418,7 @@ def measure_traffic():
         if not all(seg.values()) or not cp:
             return None
         scale = (1 << 30) / (sum(cp) / len(cp) * 1024.0)
-        kib = sum(sum(v) / len(v) for v in seg.values())  # one fold1Seg call: its three kernels
+        kib = sum(sum(v) / len(v) for v in seg.values())  # one fold1Seg call: its kernels
         vals[counter] = kib * 1024.0 * scale
         log(f"{counter}: {kib:.0f} KiB per fold1Seg call, scale {scale:.3f}")
     return int(vals["FETCH_SIZE"] + vals["WRITE_SIZE"])
@@ -472,10 +476,15 @@ def main():
     d_max = A.DeviceArray(N_SEG, np.float32)
     log(f"inputs ready ({time.time() - t0:.1f}s)")
 
+    # The step: the offsets (`scanl (+) 0` of the lengths), their tile plan
+    # (built once, shared by the two folds over those segments), the folds.
+    plan = A.SegPlan(0)
+
     def step():
         A.scan(s, A.ADD, A.LEFT_TO_RIGHT, np.int64(0), d_lens, 1, N_SEG, d_off)
-        A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)
-        A.fold_seg(s, A.MAX, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_max)
+        plan.build(s, np.float32, d_vals, N_VALUES, d_off, N_SEG)
+        plan.fold_seg(s, A.ADD, None, d_vals, 1, d_sum)
+        plan.fold_seg(s, A.MAX, None, d_vals, 1, d_max)
 
     step_bytes = (N_SEG * 8 + (N_SEG + 1) * 8) + 2 * seg_bytes(N_VALUES, N_SEG)
 
@@ -522,12 +531,21 @@ def main():
 
     # ---------------------------------------------- dominant kernel roofline
     k_bytes = seg_bytes(N_VALUES, N_SEG)
-    k_fn = lambda: A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)
-    k_ms = timed_run(s, k_fn, 50)  # mean over 50 back-to-back calls
+    k_fn = lambda: plan.fold_seg(s, A.ADD, None, d_vals, 1, d_sum)
+    u_fn = lambda: A.fold_seg(s, A.ADD, None, d_vals, 1, N_VALUES, d_off, N_SEG, d_sum)  # builds its own plan
+    # back-to-back calls, the planned and the unplanned call in alternating
+    # runs of 25 (the device's clocks drift over a run of tens of ms: the same
+    # kernel slows by up to ~15%, so an order effect would bias either one)
+    k_runs, u_runs = [], []
+    for _ in range(3):
+        k_runs.append(timed_run(s, k_fn, 25))
+        u_runs.append(timed_run(s, u_fn, 25))
+    k_ms, u_ms = float(np.mean(k_runs)), float(np.mean(u_runs))
     k_med = timed(s, k_fn, 20)     # median of 20 calls, each between its own two events
     k_gbs = k_bytes / (k_ms * 1e-3) / 1e9
+    b_ms = timed_run(s, lambda: plan.build(s, np.float32, d_vals, N_VALUES, d_off, N_SEG), 50)
     roofline = {
-        "kernel": "fold1Seg (+) Float32 call: k_tile_seg + k_fold_seg + k_seg_fix",
+        "kernel": "fold1Seg (+) Float32 call on the offsets' tile plan: k_fold_seg + k_seg_fix",
         "bound": "hbm",
         "achieved": round(k_gbs, 1),
         "peak": HBM_PEAK_GBS,
@@ -537,8 +555,11 @@ def main():
         "algorithmic_bytes": k_bytes,
         "avg_us": round(k_ms * 1e3, 2),
         "median_us": round(k_med * 1e3, 2),
+        "plan_build_us": round(b_ms * 1e3, 2),
+        "unplanned_call_us": round(u_ms * 1e3, 2),
     }
-    log(f"fold1Seg (+) kernel {k_ms * 1e3:.1f} us -> {k_gbs:.0f} GB/s")
+    log(f"fold1Seg (+) call on a plan {k_ms * 1e3:.1f} us -> {k_gbs:.0f} GB/s; plan build {b_ms * 1e3:.1f} us; "
+        f"unplanned call {u_ms * 1e3:.1f} us")
     if traffic:
         roofline["traffic"] = traffic
         if roofline["traffic"]:
@@ -680,7 +701,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "Float32 fold1Seg (+) and fold1Seg max over 2^26 N(0,1) values in 2^16 segments "
-                        "(lengths log-uniform in [1,16384], sum 2^26) + Int64 scanl (+) 0 of the lengths",
+                        "(lengths log-uniform in [1,16384], sum 2^26) + Int64 scanl (+) 0 of the lengths "
+                        "(one tile plan of the offsets per step, shared by the two folds)",
             "elements": N_VALUES,
             "segments": N_SEG,
             "seed": SEED,

@@ -345,6 +345,10 @@ class Stream:
     def synchronize(self) -> None:
         check(lib().acc_hip_stream_synchronize(self.handle))
 
+    def wait_event(self, event: "Event") -> None:
+        """Later work on this stream waits for `event` (acc_hip_stream_wait_event)."""
+        check(lib().acc_hip_stream_wait_event(self.handle, event.handle))
+
     def destroy(self) -> None:
         if self.handle:
             lib().acc_hip_stream_destroy(self.handle)

@@ -152,3 +152,21 @@ def test_c2_size_planned_equals_unplanned(stream):
         got = planned(stream, plan, op, None, d, 1, lens.size, np.float32)
         assert np.array_equal(got, unplanned(stream, op, None, d, 1, x.size, d_off, lens.size, np.float32))
     plan.destroy()
+
+
+def test_empty_inputs(stream):
+    """No elements: every segment is empty (the seed, or untouched without one);
+    no segments: nothing to do."""
+    plan = A.SegPlan(0)
+    d = A.DeviceArray(4, np.float32)
+    off = A.DeviceArray.from_numpy(stream, np.zeros(6, np.int64))
+    plan.build(stream, np.float32, d, 0, off, 5)
+    got = planned(stream, plan, A.ADD, np.float32(2.5), d, 1, 5, np.float32)
+    assert np.array_equal(got, np.full(5, 2.5, np.float32))
+    got = planned(stream, plan, A.MAX, None, d, 1, 5, np.float32)
+    assert np.array_equal(got, np.full(5, 7, np.float32))
+    plan.build(stream, np.float32, d, 4, off, 0)
+    o = A.DeviceArray.from_numpy(stream, np.full(1, 7, np.float32))
+    plan.fold_seg(stream, A.ADD, np.float32(1), d, 1, o)
+    assert o.to_numpy(stream)[0] == 7
+    plan.destroy()

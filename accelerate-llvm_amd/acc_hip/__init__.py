@@ -233,6 +233,8 @@ SIGNATURES = {
                                          c_void_p, c_int64, c_void_p]),
     "acc_hip_seg_plan_create": (c_int, [c_int, POINTER(c_void_p)]),
     "acc_hip_seg_plan_build": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_int64]),
+    "acc_hip_seg_plan_build_lengths": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_int64,
+                                               c_void_p]),
     "acc_hip_fold_seg_planned": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "acc_hip_seg_plan_destroy": (c_int, [c_void_p]),
     "acc_hip_scan": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
@@ -495,6 +497,11 @@ class SegPlan:
         of dtype whose first row starts at src."""
         check(lib().acc_hip_seg_plan_build(stream.handle, self.handle, acc_type(dtype), _p(src), n,
                                            acc_type(offsets.dtype), _p(offsets), nseg))
+
+    def build_lengths(self, stream, dtype, src, n, lengths, nseg, offsets):
+        """offsets[0..nseg] = scanl (+) 0 lengths, and the plan over them, in one launch."""
+        check(lib().acc_hip_seg_plan_build_lengths(stream.handle, self.handle, acc_type(dtype), _p(src), n,
+                                                   acc_type(lengths.dtype), _p(lengths), nseg, _p(offsets)))
 
     def fold_seg(self, stream, op, seed, src, outer, out):
         keep, sp = _scalar(src.dtype, seed)

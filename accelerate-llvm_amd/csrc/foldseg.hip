@@ -166,6 +166,123 @@ __global__ __launch_bounds__(256) void k_tile_seg(const I* off, int64_t nseg, in
   }
 }
 
+// The offsets (`scanl (+) 0` of the segment lengths) and their tile plan in
+// one launch.  A single-pass scan of the lengths: each workgroup takes a tile
+// of SP_LT lengths by ticket, scans it, and gets its prefix by the decoupled
+// look-back; it then holds the offsets of its segments g0 .. g0+SP_LT-1 and,
+// from the SLOT lengths after its tile, of the SLOT segments after them, in
+// LDS.  The tile boundaries whose first segment (k_tile_seg's first[]) lies
+// in its segments are exactly those at positions from its first offset up to
+// (not including) the next tile's first offset, so it writes their first[]
+// (a binary search in LDS) and each one's starts up to the first past that
+// fold tile (all a fold tile reads of them).  So every boundary is written
+// once, and a segment spanning many fold tiles costs two slot words per tile.
+constexpr int SP_BLOCK = 256, SP_PER = 4, SP_LT = SP_BLOCK * SP_PER;
+
+template <typename I> struct ScanPlanArgs {
+  const I* len;
+  I* off;  // nseg + 1
+  int64_t nseg;
+  int64_t tile, ntiles, shift;  // the fold's tiles
+  int64_t* first;
+  int32_t* slots;
+  SyncCtl* ctl;
+  uint64_t* status;
+};
+
+template <typename I>
+__global__ __launch_bounds__(SP_BLOCK) void k_scan_plan(ScanPlanArgs<I> a) {
+  constexpr int W = SP_BLOCK / 64;
+  constexpr int NW = Tagged<I>::NW;
+  __shared__ int64_t s_off[SP_LT + SLOT];
+  __shared__ I s_wave[W];
+  __shared__ I s_prefix;
+  __shared__ uint64_t s_tick;
+  OpAdd add{};
+  auto sub = [](I u, I v) {  // wrapping, as the scan's additions
+    using U = typename unsigned_of<I>::type;
+    return (I)((U)u - (U)v);
+  };
+  const int lane = lane_id(), w = threadIdx.x / 64;
+  if (threadIdx.x == 0) s_tick = take_ticket(a.ctl);
+  __syncthreads();
+  const uint32_t epoch = (uint32_t)(s_tick >> 32);
+  const uint32_t j = (uint32_t)s_tick;  // one ticket per workgroup
+  const int64_t g0 = (int64_t)j * SP_LT;
+  // this thread's lengths (none past the last segment) and one of the SLOT after the tile
+  I x[SP_PER];
+#pragma unroll
+  for (int k = 0; k < SP_PER; ++k) {
+    const int64_t g = g0 + threadIdx.x * SP_PER + k;
+    x[k] = g < a.nseg ? a.len[g] : (I)0;
+  }
+  I xe = 0;
+  if (threadIdx.x < SLOT && g0 + SP_LT + threadIdx.x < a.nseg) xe = a.len[g0 + SP_LT + threadIdx.x];
+  I tot = x[0];
+#pragma unroll
+  for (int k = 1; k < SP_PER; ++k) tot = add(tot, x[k]);
+  const I inc = wave_inclusive_scan(tot, add);
+  if (lane == 63) s_wave[w] = inc;
+  const I e_inc = wave_inclusive_scan(xe, add);  // the lengths after the tile (wave 0's lanes)
+  __syncthreads();
+  I wp = 0, agg = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    if (i < w) wp = add(wp, s_wave[i]);
+    agg = add(agg, s_wave[i]);
+  }
+  if (w == 0) {
+    I X = 0;
+    uint64_t* slot = a.status + (size_t)j * NW;
+    const bool last = j + 1 == gridDim.x;
+    if (j == 0) {
+      if (lane == 0 && !last) Tagged<I>::publish(slot, agg, make_tag(epoch, ST_PREFIX));
+    } else {
+      if (lane == 0) Tagged<I>::publish(slot, agg, make_tag(epoch, ST_AGGREGATE));
+      X = wave_lookback<I>(a.status, j, epoch, add);
+      if (lane == 0 && !last) Tagged<I>::publish(slot, add(X, agg), make_tag(epoch, ST_PREFIX));
+    }
+    if (lane == 0) s_prefix = X;
+  }
+  __syncthreads();
+  const I P = s_prefix;
+  I o = add(P, add(wp, sub(inc, tot)));  // offset of this thread's first segment
+#pragma unroll
+  for (int k = 0; k < SP_PER; ++k) {
+    const int idx = threadIdx.x * SP_PER + k;
+    if (g0 + idx <= a.nseg) a.off[g0 + idx] = o;
+    s_off[idx] = (int64_t)o;
+    o = add(o, x[k]);
+  }
+  if (threadIdx.x < SLOT) s_off[SP_LT + threadIdx.x] = (int64_t)add(add(P, agg), sub(e_inc, xe));
+  __syncthreads();
+  // the plan entries of the boundaries whose first segment is one of g0 .. g1-1
+  const int64_t g1 = min(g0 + SP_LT, a.nseg + 1);
+  const int64_t lo_pos = s_off[0] + a.shift, hi_pos = s_off[g1 - g0] + a.shift;
+  const int64_t b_lo = g0 == 0 ? 0 : (lo_pos <= 0 ? 0 : (lo_pos + a.tile - 1) / a.tile);
+  int64_t b_hi = g1 > a.nseg ? a.ntiles : (hi_pos <= 0 ? -1 : (hi_pos + a.tile - 1) / a.tile - 1);
+  b_hi = min(b_hi, a.ntiles);
+  for (int64_t b = b_lo + threadIdx.x; b <= b_hi; b += SP_BLOCK) {
+    const int64_t e = b * a.tile;
+    int L = 0, R = (int)(g1 - g0) - 1;  // last segment starting at or before e (g0 when none)
+    while (L < R) {
+      const int m = (L + R + 1) >> 1;
+      if (s_off[m] + a.shift <= e) L = m;
+      else R = m - 1;
+    }
+    const int64_t g = g0 + L;
+    a.first[b] = g;
+    if (b < a.ntiles) {
+      for (int i = 0; i < SLOT; ++i) {
+        const int32_t r = g + i > a.nseg ? (int32_t)a.tile + 1 : rel_pos(s_off[L + i] + a.shift, e, a.tile);
+        a.slots[(size_t)b * SLOT + i] = r;
+        if (r > a.tile) break;
+      }
+    }
+  }
+  block_retire(a.ctl, gridDim.x, epoch);
+}
+
 // The seed into every empty segment (a call on a tile plan built ahead, whose
 // pre-pass did not see the output).
 template <typename T, typename I>
@@ -230,8 +347,10 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   int64_t f0 = 0, f1 = 0;
   int32_t my_rel = 0;
   if constexpr (!UNIFORM) {
-    f0 = a.st.first[t];
-    f1 = a.st.first[t + 1];
+    // (clamped: offsets that are not ascending give wrong results, never
+    // accesses outside the offsets)
+    f0 = min(max(a.st.first[t], (int64_t)0), a.nseg);
+    f1 = min(max(a.st.first[t + 1], f0), a.nseg);
     my_rel = a.st.slots[(size_t)t * SLOT + lane];
   }
   T v[K][VEC];
@@ -754,6 +873,20 @@ extern "C" int acc_hip_seg_plan_destroy(acc_segplan_t plan) {
   return rc;
 }
 
+static int plan_reserve(acc_segplan_t plan, size_t bytes) {
+  if (bytes <= plan->cap) return ACC_SUCCESS;
+  if (plan->buf) {
+    // folds on any stream may still read the old plan
+    ACC_HIP_CHECK(hipDeviceSynchronize());
+    ACC_CHECK(dev_free(plan->device, plan->buf));
+    plan->buf = nullptr;
+    plan->cap = 0;
+  }
+  ACC_CHECK(dev_malloc(plan->device, bytes, &plan->buf));
+  plan->cap = bytes;
+  return ACC_SUCCESS;
+}
+
 extern "C" int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in, int64_t n,
                                       acc_type offset_type, const void* offsets, int64_t nseg) {
   if (!s || !plan) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: null argument");
@@ -775,17 +908,7 @@ extern "C" int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_ty
     if (ntiles >= 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "segment plan: too many elements");
     const size_t first_bytes = ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256;
     const size_t bytes = first_bytes + (size_t)ntiles * SLOT * sizeof(int32_t);
-    if (bytes > plan->cap) {
-      if (plan->buf) {
-        // folds on any stream may still read the old plan
-        ACC_HIP_CHECK(hipDeviceSynchronize());
-        ACC_CHECK(dev_free(plan->device, plan->buf));
-        plan->buf = nullptr;
-        plan->cap = 0;
-      }
-      ACC_CHECK(dev_malloc(plan->device, bytes, &plan->buf));
-      plan->cap = bytes;
-    }
+    ACC_CHECK(plan_reserve(plan, bytes));
     if (ntiles > 0 && nseg > 0)
       ACC_CHECK(launch_tile_plan<T>(s, offsets, offset_type == ACC_INT32 ? 4 : 8, nseg, TILE, ntiles, shift,
                                     (int64_t*)plan->buf, (int32_t*)((char*)plan->buf + first_bytes), nullptr,
@@ -796,6 +919,58 @@ extern "C" int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_ty
     plan->shift = shift;
     plan->ntiles = ntiles;
     plan->offset_type = offset_type;
+    plan->offsets = offsets;
+    plan->built = true;
+    return (int)ACC_SUCCESS;
+  });
+}
+
+extern "C" int acc_hip_seg_plan_build_lengths(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in,
+                                              int64_t n, acc_type len_type, const void* lengths, int64_t nseg,
+                                              void* offsets) {
+  if (!s || !plan) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: null argument");
+  if (s->device != plan->device) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: stream on another device");
+  if (n < 0 || nseg < 0) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: negative extent");
+  if (len_type != ACC_INT32 && len_type != ACC_INT64)
+    return fail(ACC_ERROR_UNSUPPORTED, "segment plan: lengths must be Int32 or Int64");
+  const size_t B = type_size(t);
+  if (!B) return fail(ACC_ERROR_UNSUPPORTED, "segment plan: element type %d", (int)t);
+  if (!offsets || (nseg > 0 && !lengths)) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: null array");
+  if ((uintptr_t)in % B) return fail(ACC_ERROR_INVALID_VALUE, "segment plan: input not element-aligned");
+  if (nseg >= ((int64_t)1 << 40)) return fail(ACC_ERROR_UNSUPPORTED, "segment plan: too many segments");
+  ACC_CHECK(set_device(s->device));
+  plan->built = false;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    constexpr int64_t TILE = (int64_t)SegTune<T>::BLOCK * SegTune<T>::K * Vec<T>::N;
+    const int64_t shift = (int64_t)(((uintptr_t)in % 16) / sizeof(T));
+    const int64_t ntiles = n == 0 ? 0 : ceil_div(n + shift, TILE);
+    if (ntiles >= 0x7fffffffLL) return fail(ACC_ERROR_UNSUPPORTED, "segment plan: too many elements");
+    const size_t first_bytes = ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256;
+    ACC_CHECK(plan_reserve(plan, first_bytes + (size_t)ntiles * SLOT * sizeof(int32_t)));
+    const int64_t ltiles = ceil_div(nseg + 1, (int64_t)SP_LT);
+    const size_t I = len_type == ACC_INT32 ? 4 : 8;
+    SyncCtl* ctl;
+    uint64_t* status;
+    ACC_CHECK(stream_workspace(s, (size_t)ltiles * (I == 4 ? 1 : 2) * 8, &ctl, &status));
+    int64_t* first = (int64_t*)plan->buf;
+    int32_t* slots = (int32_t*)((char*)plan->buf + first_bytes);
+    if (I == 4) {
+      ScanPlanArgs<int32_t> a{(const int32_t*)lengths, (int32_t*)offsets, nseg, TILE, ntiles, shift, first, slots,
+                              ctl, status};
+      hipLaunchKernelGGL((k_scan_plan<int32_t>), dim3((unsigned)ltiles), dim3(SP_BLOCK), 0, s->stream, a);
+    } else {
+      ScanPlanArgs<int64_t> a{(const int64_t*)lengths, (int64_t*)offsets, nseg, TILE, ntiles, shift, first, slots,
+                              ctl, status};
+      hipLaunchKernelGGL((k_scan_plan<int64_t>), dim3((unsigned)ltiles), dim3(SP_BLOCK), 0, s->stream, a);
+    }
+    ACC_HIP_CHECK(hipGetLastError());
+    plan->type = t;
+    plan->n = n;
+    plan->nseg = nseg;
+    plan->shift = shift;
+    plan->ntiles = ntiles;
+    plan->offset_type = len_type;
     plan->offsets = offsets;
     plan->built = true;
     return (int)ACC_SUCCESS;

@@ -245,6 +245,13 @@ typedef struct acc_segplan_s* acc_segplan_t;
 int acc_hip_seg_plan_create(int device, acc_segplan_t* plan);
 int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in, int64_t n,
                            acc_type offset_type, const void* offsets, int64_t nseg);
+/* The same plan from the segment lengths, with the offsets: writes
+ * offsets[0 .. nseg] = scanl (+) 0 lengths (Int32 or Int64, wrapping like the
+ * scan) and plans over them, in one launch: the frontend's
+ * `foldSeg' f z xs (scanl (+) 0 seg)` producer and the plan together.  The
+ * plan then refers to `offsets`. */
+int acc_hip_seg_plan_build_lengths(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in, int64_t n,
+                                   acc_type len_type, const void* lengths, int64_t nseg, void* offsets);
 int acc_hip_fold_seg_planned(acc_stream_t s, acc_segplan_t plan, acc_type t, acc_op op, const void* seed,
                              const void* in, int64_t outer, void* out);
 int acc_hip_seg_plan_destroy(acc_segplan_t plan);

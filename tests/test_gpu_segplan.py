@@ -170,3 +170,72 @@ def test_empty_inputs(stream):
     plan.fold_seg(stream, A.ADD, np.float32(1), d, 1, o)
     assert o.to_numpy(stream)[0] == 7
     plan.destroy()
+
+
+def lengths_case(case, rng):
+    if case == "mixed":
+        from acc_hip import workloads as W
+        return W.segment_lengths(3, 3000, 16384, 2_000_003)
+    if case == "long":  # segments over hundreds of fold tiles, empty ones between
+        return np.array([5, 3_000_000, 0, 7, 1_000_000, 0, 0, 12], np.int64)
+    if case == "tiny":  # more than 64 starts per fold tile, many empty segments
+        return rng.integers(0, 3, size=50000).astype(np.int64)
+    if case == "tile_edge":  # nseg a multiple of the scan's tile, and one more
+        return rng.integers(1, 100, size=2048).astype(np.int64)
+    return rng.integers(1, 100, size=2049).astype(np.int64)
+
+
+@pytest.mark.parametrize("lt", [np.int32, np.int64])
+@pytest.mark.parametrize("case", ["mixed", "long", "tiny", "tile_edge", "tile_edge_plus1"])
+def test_build_from_lengths(stream, lt, case):
+    """scanl (+) 0 of the lengths and the plan in one launch: the offsets equal
+    numpy's, and folds on that plan equal the unplanned folds over them."""
+    rng = np.random.default_rng(len(case) * 7 + (lt == np.int64))
+    lens = lengths_case(case, rng)
+    nseg, n = lens.size, int(lens.sum())
+    x = data(rng, np.float32, n + 3)
+    base = A.DeviceArray.from_numpy(stream, x)
+    d = base.view(3, n) if case == "mixed" else base.view(0, n)  # a view 12 bytes past a 16-byte boundary
+    xs = x[3:3 + n] if case == "mixed" else x[:n]
+    d_len = A.DeviceArray.from_numpy(stream, lens.astype(lt))
+    d_off = A.DeviceArray(nseg + 1, lt)
+    plan = A.SegPlan(0)
+    plan.build_lengths(stream, np.float32, d, n, d_len, nseg, d_off)
+    off = d_off.to_numpy(stream)
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(lens)]).astype(lt))
+    for op, seed in ((A.ADD, None), (A.MAX, None), (A.ADD, np.float32(1.5))):
+        got = planned(stream, plan, op, seed, d, 1, nseg, np.float32)
+        assert np.array_equal(got, unplanned(stream, op, seed, d, 1, n, d_off, nseg, np.float32)), (op, seed)
+        against_oracle(op, seed, xs, off.astype(np.int64), got)
+    plan.destroy()
+
+
+def test_build_from_lengths_offsets_wrap_like_the_scan(stream):
+    """Int32 lengths whose sum passes 2^31: the offsets wrap as the scan's do."""
+    lens = np.full(3000, 1_000_000, np.int32)
+    d_len = A.DeviceArray.from_numpy(stream, lens)
+    d_off = A.DeviceArray(lens.size + 1, np.int32)
+    want = A.DeviceArray(lens.size + 1, np.int32)
+    A.scan(stream, A.ADD, A.LEFT_TO_RIGHT, np.int32(0), d_len, 1, lens.size, want)
+    plan = A.SegPlan(0)
+    plan.build_lengths(stream, np.float32, A.DeviceArray(16, np.float32), 16, d_len, lens.size, d_off)
+    assert np.array_equal(d_off.to_numpy(stream), want.to_numpy(stream))
+    assert np.array_equal(d_off.to_numpy(stream), np.concatenate([[0], np.cumsum(lens, dtype=np.int32)]))
+    plan.destroy()
+
+
+def test_c2_step_from_lengths(stream):
+    """The benchmark step's shape through the one-launch offsets and plan."""
+    from acc_hip import workloads as W
+    lens = W.segment_lengths(42, 1 << 16, 16384, 1 << 26)
+    x = W.normal_f32(42, 1 << 26)
+    d, d_len = A.DeviceArray.from_numpy(stream, x), A.DeviceArray.from_numpy(stream, lens)
+    d_off = A.DeviceArray(lens.size + 1, np.int64)
+    plan = A.SegPlan(0)
+    for _ in range(2):  # a rebuild on the same plan
+        plan.build_lengths(stream, np.float32, d, x.size, d_len, lens.size, d_off)
+        assert np.array_equal(d_off.to_numpy(stream), W.offsets(lens))
+        for op in (A.ADD, A.MAX):
+            got = planned(stream, plan, op, None, d, 1, lens.size, np.float32)
+            assert np.array_equal(got, unplanned(stream, op, None, d, 1, x.size, d_off, lens.size, np.float32))
+    plan.destroy()

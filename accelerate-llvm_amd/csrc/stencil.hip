@@ -8,7 +8,7 @@
 // (TH+2RY) x (TW+2RX) halo tile in LDS (interior tiles with dwordx4 copies,
 // border tiles applying the boundary map, LLVM/CodeGen/Stencil.hs:224-253, to
 // the coordinates outside the array as they load), then each thread computes
-// a 4x4 block of outputs from a register window read from LDS with 16-byte
+// an 8x2 block of outputs from a register window read from LDS with 16-byte
 // reads and writes it with dwordx4 stores.
 //
 // The stencil function is f(tuple) = sum_{dy,dx} w[dy][dx] * x[y+dy][x+dx]
@@ -24,10 +24,14 @@
 
 namespace acc {
 
-constexpr int ST_TW = 64;  // tile width (columns)
-constexpr int ST_TH = 64;  // tile height (rows)
-constexpr int ST_PX = 4;   // outputs per thread in x
-constexpr int ST_PY = 4;   // outputs per thread in y
+// Wide, flat tiles with two 16-byte output vectors per thread and row:
+// 8192^2 Float32 3x3 106 -> 91 us and 5x5 109 -> 95 us against 64 x 64 tiles
+// of 4 x 4 outputs (128 x 32 and 256 x 32 tiles of 8 x 2 or 8 x 4 outputs were
+// within 2 us of this; 128 x 64 of 4 x 8 outputs no better than 64 x 64).
+constexpr int ST_TW = 256;  // tile width (columns)
+constexpr int ST_TH = 16;   // tile height (rows)
+constexpr int ST_PX = 8;    // outputs per thread in x
+constexpr int ST_PY = 2;    // outputs per thread in y
 constexpr int ST_BLOCK = (ST_TW / ST_PX) * (ST_TH / ST_PY);  // 256
 
 template <typename T> struct StArgs {
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(ST_BLOCK) void k_stencil2d(StArgs<T> a) {
     __syncthreads();
   }
 
-  // ---- 4x4 outputs per thread from a register window
+  // ---- ST_PX x ST_PY outputs per thread from a register window
   const int px = threadIdx.x % (ST_TW / ST_PX), py = threadIdx.x / (ST_TW / ST_PX);
   const int cx = px * ST_PX, cy = py * ST_PY;  // tile-local output origin
   T acc[ST_PY][ST_PX];

@@ -240,7 +240,8 @@ int acc_hip_fold_seg(acc_stream_t s, acc_type t, acc_op op, const void* seed, co
  * other than the plan's (with outer > 1, rows must be 16-byte multiples).
  * The offsets must not change while the plan is used, and a fold on another
  * stream than the build's must be ordered after it (an event).  A build that
- * needs a larger plan synchronises the device before freeing the old one. */
+ * needs a larger plan synchronises the device before freeing the old one.  A
+ * plan is not locked: no thread may build it while another uses it. */
 typedef struct acc_segplan_s* acc_segplan_t;
 int acc_hip_seg_plan_create(int device, acc_segplan_t* plan);
 int acc_hip_seg_plan_build(acc_stream_t s, acc_segplan_t plan, acc_type t, const void* in, int64_t n,

@@ -709,7 +709,9 @@ def test_stencil_float64_special(stream):
 
 
 # ------------------------------------------------------------------ 3-D stencil
-S3_SHAPES = [(1, 1, 1), (2, 3, 4), (5, 17, 70), (33, 40, 130), (70, 20, 64)]
+# (6, 40, 300) and (5, 37, 520) have interior blocks (16-byte vector staging)
+# beside the border blocks; the others are border blocks only
+S3_SHAPES = [(1, 1, 1), (2, 3, 4), (5, 17, 70), (33, 40, 130), (70, 20, 64), (6, 40, 300), (5, 37, 520)]
 
 
 @pytest.mark.parametrize("shape", S3_SHAPES)
@@ -729,7 +731,7 @@ def test_stencil3d_int_exact(stream, dt, r, shape):
         assert np.array_equal(got, O.stencil3d(rz, ry, rx, wt, b, dt(5), vol)), (b, shape)
 
 
-@pytest.mark.parametrize("shape", [(9, 33, 65), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(9, 33, 65), (64, 64, 64), (9, 50, 520)])
 @pytest.mark.parametrize("r", [(1, 1, 1), (2, 2, 2)])
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_stencil3d_float_bound(stream, dt, r, shape):
@@ -748,13 +750,14 @@ def test_stencil3d_float_bound(stream, dt, r, shape):
 
 
 def test_stencil3d_large_volume(stream):
-    # 256^3 Float32, 3x3x3 clamp: several z runs per column of blocks
-    n = 256
-    vol = O.gen_normal_f32(9, n ** 3).reshape(n, n, n)
+    # 256 x 256 x 384 Float32, 3x3x3 clamp: several z runs per column of
+    # blocks, interior and border blocks
+    d, h, w = 256, 256, 384
+    vol = O.gen_normal_f32(9, d * h * w).reshape(d, h, w)
     wt = np.full((3, 3, 3), 1.0 / 27.0, np.float32)
     out = A.DeviceArray(vol.size, np.float32)
-    A.stencil3d(stream, 1, 1, 1, wt, CLAMP, None, dev(stream, vol), n, n, n, out)
-    got = out.to_numpy(stream).reshape(n, n, n).astype(np.float64)
+    A.stencil3d(stream, 1, 1, 1, wt, CLAMP, None, dev(stream, vol), d, h, w, out)
+    got = out.to_numpy(stream).reshape(d, h, w).astype(np.float64)
     truth, ab = O.stencil3d_f64(1, 1, 1, wt, CLAMP, None, vol)
     assert np.all(np.abs(got - truth) <= 27 * 2.0 ** -23 * ab + 1e-30)
 

@@ -709,8 +709,7 @@ def test_stencil_float64_special(stream):
 
 
 # ------------------------------------------------------------------ 3-D stencil
-# (6, 40, 300) and (5, 37, 520) have interior blocks (16-byte vector staging)
-# beside the border blocks; the others are border blocks only
+# (6, 40, 300) and (5, 37, 520): rows of several blocks, 16-byte aligned
 S3_SHAPES = [(1, 1, 1), (2, 3, 4), (5, 17, 70), (33, 40, 130), (70, 20, 64), (6, 40, 300), (5, 37, 520)]
 
 
@@ -751,7 +750,7 @@ def test_stencil3d_float_bound(stream, dt, r, shape):
 
 def test_stencil3d_large_volume(stream):
     # 256 x 256 x 384 Float32, 3x3x3 clamp: several z runs per column of
-    # blocks, interior and border blocks
+    # blocks, several blocks per row
     d, h, w = 256, 256, 384
     vol = O.gen_normal_f32(9, d * h * w).reshape(d, h, w)
     wt = np.full((3, 3, 3), 1.0 / 27.0, np.float32)

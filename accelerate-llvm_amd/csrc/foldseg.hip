@@ -273,10 +273,17 @@ __global__ __launch_bounds__(SP_BLOCK) void k_scan_plan(ScanPlanArgs<I> a) {
     const int64_t g = g0 + L;
     a.first[b] = g;
     if (b < a.ntiles) {
-      for (int i = 0; i < SLOT; ++i) {
-        const int32_t r = g + i > a.nseg ? (int32_t)a.tile + 1 : rel_pos(s_off[L + i] + a.shift, e, a.tile);
-        a.slots[(size_t)b * SLOT + i] = r;
-        if (r > a.tile) break;
+      // four starts per 16-byte store (rows are 256-byte aligned), up to the
+      // first past the fold tile
+      for (int i = 0; i < SLOT; i += 4) {
+        int32_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          r[k] = g + i + k > a.nseg ? (int32_t)a.tile + 1 : rel_pos(s_off[L + i + k] + a.shift, e, a.tile);
+        typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+        const i32x4 v = {r[0], r[1], r[2], r[3]};
+        *(__attribute__((address_space(1))) i32x4*)(a.slots + (size_t)b * SLOT + i) = v;
+        if (r[3] > a.tile) break;
       }
     }
   }

@@ -236,6 +236,8 @@ SIGNATURES = {
     "acc_hip_seg_plan_build_lengths": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_int, c_void_p, c_int64,
                                                c_void_p]),
     "acc_hip_fold_seg_planned": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "acc_hip_fold_seg_planned_multi": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                               c_void_p]),
     "acc_hip_seg_plan_destroy": (c_int, [c_void_p]),
     "acc_hip_scan": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p]),
     "acc_hip_scan_prime": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int64, c_int64, c_void_p,
@@ -507,6 +509,17 @@ class SegPlan:
         keep, sp = _scalar(src.dtype, seed)
         check(lib().acc_hip_fold_seg_planned(stream.handle, self.handle, acc_type(src.dtype), op, sp, _p(src), outer,
                                              _p(out)))
+
+    def fold_seg_multi(self, stream, folds):
+        """Several foldSegs on this plan in one call: folds = [(op, seed_or_None, src, out), ...]."""
+        k = len(folds)
+        ops = (c_int * k)(*[f[0] for f in folds])
+        keep = [_scalar(f[2].dtype, f[1]) for f in folds]
+        seeds = (c_void_p * k)(*[kp[1] for kp in keep])
+        ins = (c_void_p * k)(*[_p(f[2]) for f in folds])
+        outs = (c_void_p * k)(*[_p(f[3]) for f in folds])
+        check(lib().acc_hip_fold_seg_planned_multi(stream.handle, self.handle, acc_type(folds[0][2].dtype), k, ops,
+                                                   seeds, ins, outs))
 
     def destroy(self):
         if self.handle:

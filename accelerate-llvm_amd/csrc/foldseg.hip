@@ -615,25 +615,23 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
 // thread (coalesced), scans them, publishes its aggregate and takes the
 // carry from the workgroups before it with a decoupled look-back (with at
 // most 64 workgroups one window covers them all).
+// One workgroup's part of the fix-up: tiles grp*BLOCK .. +BLOCK-1 of a
+// foldSeg whose look-back status words start at `first` (status slot
+// first + grp).
 template <typename T, typename Op, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_seg_fix(CarryArrays<T> carry, uint32_t ntiles, T* out,
-                                                   int64_t nseg, uint32_t has_seed, T seed, SyncCtl* ctl,
-                                                   uint64_t* status) {
+__device__ __forceinline__ void seg_fix_group(const CarryArrays<T>& carry, uint32_t ntiles, T* out, int64_t nseg,
+                                              uint32_t has_seed, T seed, uint64_t* status, uint32_t first,
+                                              uint32_t grp, uint32_t ngrp, uint32_t epoch) {
   constexpr int W = BLOCK / 64;
   using S = SegV<T>;
   constexpr int NW = Tagged<S>::NW;
-  __shared__ uint64_t s_tick;
   __shared__ S s_part[W];
   __shared__ S s_carry;
   SegOp<Op> sop{Op{}};
   Op op{};
   const S sid{identity<T>(op), 0u};
   const int lane = lane_id(), w = threadIdx.x / 64;
-  if (threadIdx.x == 0) s_tick = take_ticket(ctl);
-  __syncthreads();
-  const uint32_t epoch = (uint32_t)(s_tick >> 32);
-  const uint32_t b = (uint32_t)s_tick;  // < gridDim.x: one ticket per workgroup
-  const uint32_t t = b * BLOCK + threadIdx.x;
+  const uint32_t t = grp * BLOCK + threadIdx.x;
   TileCarry<T> c;
   if (t < ntiles) c = carry.get(t);
   else c = TileCarry<T>{sid, sid.v, -1};
@@ -650,11 +648,12 @@ __global__ __launch_bounds__(BLOCK) void k_seg_fix(CarryArrays<T> carry, uint32_
   }
   if (w == 0) {
     S X = sid;
+    const uint32_t b = first + grp;
     uint64_t* slot = status + (size_t)b * NW;
-    const bool last = b + 1 == gridDim.x;
-    if (b > 0) {
+    const bool last = grp + 1 == ngrp;
+    if (grp > 0) {
       if (lane == 0) Tagged<S>::publish(slot, agg, make_tag(epoch, agg.f ? ST_PREFIX : ST_AGGREGATE));
-      X = wave_lookback<S>(status, b, epoch, sop);
+      X = wave_lookback<S>(status, b, epoch, sop, first);
       if (lane == 0 && !agg.f && !last) Tagged<S>::publish(slot, sop(X, agg), make_tag(epoch, ST_PREFIX));
     } else if (lane == 0 && !last) {
       Tagged<S>::publish(slot, agg, make_tag(epoch, ST_PREFIX));
@@ -667,6 +666,58 @@ __global__ __launch_bounds__(BLOCK) void k_seg_fix(CarryArrays<T> carry, uint32_
     T v = op(X.v, c.open_v);
     if (has_seed) v = op(seed, v);
     out[c.open_g] = v;
+  }
+}
+
+template <typename T, typename Op, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_fix(CarryArrays<T> carry, uint32_t ntiles, T* out,
+                                                   int64_t nseg, uint32_t has_seed, T seed, SyncCtl* ctl,
+                                                   uint64_t* status) {
+  __shared__ uint64_t s_tick;
+  if (threadIdx.x == 0) s_tick = take_ticket(ctl);
+  __syncthreads();
+  const uint32_t epoch = (uint32_t)(s_tick >> 32);
+  const uint32_t b = (uint32_t)s_tick;  // < gridDim.x: one ticket per workgroup
+  seg_fix_group<T, Op, BLOCK>(carry, ntiles, out, nseg, has_seed, seed, status, 0u, b, gridDim.x, epoch);
+  block_retire(ctl, gridDim.x, epoch);
+}
+
+// The fix-ups of several foldSegs over one tile plan in one launch: ticket t
+// is group t % ngrp of fold t / ngrp, each fold with its own records, output,
+// combine and status words (a look-back never crosses into another fold's).
+constexpr int kMultiFolds = 8;
+template <typename T> struct MultiFix {
+  CarryArrays<T> carry[kMultiFolds];
+  T* out[kMultiFolds];
+  T seed[kMultiFolds];
+  int32_t op[kMultiFolds];
+  uint32_t has_seed[kMultiFolds];
+};
+
+template <typename T, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_fix_multi(MultiFix<T> m, uint32_t ntiles, uint32_t ngrp, int64_t nseg,
+                                                         SyncCtl* ctl, uint64_t* status) {
+  __shared__ uint64_t s_tick;
+  if (threadIdx.x == 0) s_tick = take_ticket(ctl);
+  __syncthreads();
+  const uint32_t epoch = (uint32_t)(s_tick >> 32);
+  const uint32_t tk = (uint32_t)s_tick;  // < gridDim.x
+  const uint32_t f = tk / ngrp, grp = tk - f * ngrp;
+  const uint32_t first = f * ngrp;
+  auto go = [&](auto opv) {
+    using Op = decltype(opv);
+    seg_fix_group<T, Op, BLOCK>(m.carry[f], ntiles, m.out[f], nseg, m.has_seed[f], m.seed[f], status, first, grp,
+                                ngrp, epoch);
+  };
+  const int32_t opc = m.op[f];  // uniform
+  if (opc == ACC_OP_ADD) go(OpAdd{});
+  else if (opc == ACC_OP_MUL) go(OpMul{});
+  else if (opc == ACC_OP_MIN) go(OpMin{});
+  else if (opc == ACC_OP_MAX) go(OpMax{});
+  else if constexpr (std::is_integral<T>::value) {
+    if (opc == ACC_OP_AND) go(OpAnd{});
+    else if (opc == ACC_OP_OR) go(OpOr{});
+    else go(OpXor{});
   }
   block_retire(ctl, gridDim.x, epoch);
 }
@@ -1024,5 +1075,88 @@ extern "C" int acc_hip_fold_seg_planned(acc_stream_t s, acc_segplan_t plan, acc_
       }
       return (int)ACC_SUCCESS;
     });
+  });
+}
+
+extern "C" int acc_hip_fold_seg_planned_multi(acc_stream_t s, acc_segplan_t plan, acc_type t, int count,
+                                              const acc_op* ops, const void* const* seeds, const void* const* ins,
+                                              void* const* outs) {
+  if (!s || !plan || !ops || !ins || !outs) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): null argument");
+  if (!plan->built) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): the plan has not been built");
+  if (s->device != plan->device)
+    return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): stream on another device");
+  if (t != plan->type)
+    return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): element type differs from the plan's");
+  if (count < 1 || count > kMultiFolds)
+    return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): 1 to %d folds", kMultiFolds);
+  const size_t B = type_size(t);
+  for (int i = 0; i < count; ++i) {
+    if (!ins[i] || !outs[i]) return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): null array %d", i);
+    if ((uintptr_t)ins[i] % B || (int64_t)(((uintptr_t)ins[i] % 16) / B) != plan->shift)
+      return fail(ACC_ERROR_INVALID_VALUE, "foldSeg (planned, several): input %d alignment differs from the plan's", i);
+  }
+  const int64_t n = plan->n, nseg = plan->nseg, ntiles = plan->ntiles;
+  if (nseg == 0) return ACC_SUCCESS;
+  ACC_CHECK(set_device(s->device));
+  const size_t I = plan->offset_type == ACC_INT32 ? 4 : 8;
+  const size_t first_bytes = ((size_t)(ntiles + 1) * sizeof(int64_t) + 255) / 256 * 256;
+  const TilePlan tp{(const int64_t*)plan->buf, (const int32_t*)((const char*)plan->buf + first_bytes)};
+  const void* offsets = plan->offsets;
+  return with_type(t, [&](auto z) {
+    using T = decltype(z);
+    constexpr int BLOCK = SegTune<T>::BLOCK, K = SegTune<T>::K, CAP = SegTune<T>::CAP;
+    constexpr int FIX = 256;
+    if (ntiles == 0) {  // no elements: every segment is empty
+      for (int i = 0; i < count; ++i)
+        if (seeds && seeds[i]) ACC_CHECK(acc_hip_fill_async(s, outs[i], seeds[i], sizeof(T), (size_t)nseg));
+      return (int)ACC_SUCCESS;
+    }
+    // per fold: its own tile records (untagged scratch); one block of status
+    // words per fold for the shared fix-up
+    auto al256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t cv_bytes = al256((size_t)ntiles * sizeof(T)), cf_bytes = al256((size_t)ntiles);
+    const size_t carry_bytes = al256(2 * cv_bytes + cf_bytes + (size_t)ntiles * 8);
+    const int64_t nfix = ceil_div(ntiles, (int64_t)FIX);
+    SyncCtl* ctl;
+    uint64_t* status;
+    void* scr;
+    ACC_CHECK(stream_workspace(s, (size_t)count * nfix * Tagged<SegV<T>>::NW * 8, &ctl, &status));
+    ACC_CHECK(stream_scratch(s, (size_t)count * carry_bytes, &scr));
+    MultiFix<T> m{};
+    for (int i = 0; i < count; ++i) {
+      char* cb = (char*)scr + (size_t)i * carry_bytes;
+      m.carry[i] = CarryArrays<T>{(T*)cb, (uint8_t*)(cb + cv_bytes), (T*)(cb + cv_bytes + cf_bytes),
+                                  (int64_t*)(cb + 2 * cv_bytes + cf_bytes)};
+      m.out[i] = (T*)outs[i];
+      m.has_seed[i] = seeds && seeds[i] ? 1u : 0u;
+      m.seed[i] = m.has_seed[i] ? *(const T*)seeds[i] : T();
+      m.op[i] = (int32_t)ops[i];
+      const T* in0 = (const T*)ins[i] - plan->shift;
+      const int rc = with_op<T>(ops[i], [&](auto f) {
+        using Op = decltype(f);
+        if (m.has_seed[i]) ACC_CHECK(launch_fill_empty(s, offsets, I, nseg, m.out[i], m.seed[i]));
+        auto launch = [&](auto st) {
+          using Starts = decltype(st);
+          st.first = tp.first;
+          st.slots = tp.slots;
+          st.shift = plan->shift;
+          SegArgs<T, Starts> a{in0, m.out[i], st, nseg, plan->shift, n + plan->shift, (uint32_t)ntiles, m.has_seed[i],
+                               m.seed[i], m.carry[i]};
+          hipLaunchKernelGGL((k_fold_seg<T, Op, Starts, BLOCK, K, CAP>), dim3((unsigned)ntiles), dim3(BLOCK), 0,
+                             s->stream, a);
+          ACC_HIP_CHECK(hipGetLastError());
+          return (int)ACC_SUCCESS;
+        };
+        if (I == 4) return launch(OffsetStarts<int32_t>{(const int32_t*)offsets, nullptr, nullptr, 0});
+        return launch(OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr, nullptr, 0});
+      });
+      if (rc) return rc;
+    }
+    if (ntiles > 1) {
+      hipLaunchKernelGGL((k_seg_fix_multi<T, FIX>), dim3((unsigned)(count * nfix)), dim3(FIX), 0, s->stream, m,
+                         (uint32_t)ntiles, (uint32_t)nfix, nseg, ctl, status);
+      ACC_HIP_CHECK(hipGetLastError());
+    }
+    return (int)ACC_SUCCESS;
   });
 }

@@ -255,6 +255,13 @@ int acc_hip_seg_plan_build_lengths(acc_stream_t s, acc_segplan_t plan, acc_type 
                                    acc_type len_type, const void* lengths, int64_t nseg, void* offsets);
 int acc_hip_fold_seg_planned(acc_stream_t s, acc_segplan_t plan, acc_type t, acc_op op, const void* seed,
                              const void* in, int64_t outer, void* out);
+/* Several foldSegs (rows of one, outer = 1) over one plan in one call: fold i
+ * combines ins[i] with ops[i] (seeds[i] or NULL; seeds itself may be NULL)
+ * into outs[i], 1 <= count <= 8, every input of the plan's type and
+ * alignment.  The tile kernels run one after the other and the fix-ups of all
+ * the folds share one launch. */
+int acc_hip_fold_seg_planned_multi(acc_stream_t s, acc_segplan_t plan, acc_type t, int count, const acc_op* ops,
+                                   const void* const* seeds, const void* const* ins, void* const* outs);
 int acc_hip_seg_plan_destroy(acc_segplan_t plan);
 
 /* Allocation-free foldSeg / fold1Seg.  No counterpart in the reference: it is

@@ -239,3 +239,50 @@ def test_c2_step_from_lengths(stream):
             got = planned(stream, plan, op, None, d, 1, lens.size, np.float32)
             assert np.array_equal(got, unplanned(stream, op, None, d, 1, x.size, d_off, lens.size, np.float32))
     plan.destroy()
+
+
+@pytest.mark.parametrize("dt", TYPES)
+def test_several_folds_in_one_call(stream, dt):
+    """acc_hip_fold_seg_planned_multi: each fold bit-identical to its own
+    planned call (the tile kernels are the same; the fix-ups share a launch)."""
+    rng = np.random.default_rng(17 + TYPES.index(dt))
+    n, nseg = 700_001, 1999
+    off = segments(rng, n, nseg, empties=True)
+    d_off = A.DeviceArray.from_numpy(stream, off)
+    xs = [data(rng, dt, n) for _ in range(4)]
+    ds = [A.DeviceArray.from_numpy(stream, x) for x in xs]
+    plan = A.SegPlan(0)
+    plan.build(stream, dt, ds[0], n, d_off, nseg)
+    mul = A.MUL if np.dtype(dt).kind != "f" else A.ADD
+    spec = [(A.ADD, None), (A.MAX, dt(2)), (A.MIN, None), (mul, dt(3))]
+    outs = [A.DeviceArray.from_numpy(stream, np.full(nseg, 7, dt)) for _ in spec]
+    plan.fold_seg_multi(stream, [(op, sd, d, o) for (op, sd), d, o in zip(spec, ds, outs)])
+    for (op, sd), d, o, x in zip(spec, ds, outs, xs):
+        got = o.to_numpy(stream)
+        assert np.array_equal(got, planned(stream, plan, op, sd, d, 1, nseg, dt)), (op, sd)
+        against_oracle(op, sd, x, off, got) if op != A.MUL else None
+    plan.destroy()
+
+
+def test_several_folds_limits(stream):
+    rng = np.random.default_rng(23)
+    plan = A.SegPlan(0)
+    # eight folds of one small tile (no fix-up launch), and nine refused
+    x = data(rng, np.float32, 100)
+    off = segments(rng, 100, 9)
+    d, d_off = A.DeviceArray.from_numpy(stream, x), A.DeviceArray.from_numpy(stream, off)
+    plan.build(stream, np.float32, d, 100, d_off, 9)
+    outs = [A.DeviceArray(9, np.float32) for _ in range(9)]
+    plan.fold_seg_multi(stream, [(A.ADD if k % 2 else A.MAX, None, d, outs[k]) for k in range(8)])
+    for k in range(8):
+        op = A.ADD if k % 2 else A.MAX
+        assert np.array_equal(outs[k].to_numpy(stream), planned(stream, plan, op, None, d, 1, 9, np.float32))
+    with pytest.raises(A.AccError):
+        plan.fold_seg_multi(stream, [(A.ADD, None, d, outs[k]) for k in range(9)])
+    # no elements: the seeds
+    plan.build(stream, np.float32, d, 0, A.DeviceArray.from_numpy(stream, np.zeros(4, np.int64)), 3)
+    o1, o2 = A.DeviceArray.from_numpy(stream, np.full(3, 7, np.float32)), A.DeviceArray(3, np.float32)
+    plan.fold_seg_multi(stream, [(A.ADD, None, d, o1), (A.MAX, np.float32(-1), d, o2)])
+    assert np.array_equal(o1.to_numpy(stream), np.full(3, 7, np.float32))
+    assert np.array_equal(o2.to_numpy(stream), np.full(3, -1, np.float32))
+    plan.destroy()

@@ -428,7 +428,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=60, help="untimed steps right before the timed ones (the device settles over the first few ms of work)")
     ap.add_argument("--skeleton-reps", type=int, default=20, help="timed calls per skeleton (median reported)")
     ap.add_argument("--no-skeletons", action="store_true", help="skip the per-skeleton lines")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -478,18 +478,16 @@ def main():
 
     # The step: the offsets (`scanl (+) 0` of the lengths) together with their
     # tile plan (one launch; the plan is shared by the two folds over those
-    # segments), then the folds.
+    # segments), then the two folds in one call (their fix-ups share a launch).
     plan = A.SegPlan(0)
 
     def step():
         plan.build_lengths(s, np.float32, d_vals, N_VALUES, d_lens, N_SEG, d_off)
-        plan.fold_seg(s, A.ADD, None, d_vals, 1, d_sum)
-        plan.fold_seg(s, A.MAX, None, d_vals, 1, d_max)
+        plan.fold_seg_multi(s, [(A.ADD, None, d_vals, d_sum), (A.MAX, None, d_vals, d_max)])
 
     step_bytes = (N_SEG * 8 + (N_SEG + 1) * 8) + 2 * seg_bytes(N_VALUES, N_SEG)
 
-    for _ in range(args.warmup):
-        step()
+    step()  # one step to check the outputs of
     s.synchronize()
 
     # sanity of the step's outputs with numpy (the parity tests cover the rest)
@@ -516,6 +514,10 @@ def main():
         f"p99 {fp_error['p99_rel']:.3e}; max error/bound {fp_error['max_err_over_bound']:.3f})")
 
     # ------------------------------------------------------------ timed steps
+    # the untimed warm-up steps right before the timed ones (the checks above
+    # left the device idle, and its clocks settle over the first ms of work)
+    for _ in range(args.warmup):
+        step()
     e0, e1 = A.Event(0), A.Event(0)
     s.synchronize()
     w0 = time.perf_counter()
@@ -709,7 +711,7 @@ def main():
         "config": {
             "workload": "Float32 fold1Seg (+) and fold1Seg max over 2^26 N(0,1) values in 2^16 segments "
                         "(lengths log-uniform in [1,16384], sum 2^26) + Int64 scanl (+) 0 of the lengths "
-                        "(the offsets and their tile plan in one launch per step, the plan shared by the two folds)",
+                        "(the offsets and their tile plan in one launch per step; the two folds in one call on that plan)",
             "elements": N_VALUES,
             "segments": N_SEG,
             "seed": SEED,

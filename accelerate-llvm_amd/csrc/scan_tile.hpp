@@ -132,12 +132,17 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
       if (a.has_seed) X = a.seed;
     } else {
       if (lane == 0) Tagged<T>::publish(slot, agg, make_tag(epoch, ST_AGGREGATE));
-      // windows of 3 x 64 predecessors: the published-prefix front of a
-      // long scan then advances up to 192 tiles per round trip (measured on
-      // 2^28 Int32: 1 chunk 432 us, 2: 412 us, 3: 401-408 us, 4: 417-426 us)
+      // a first window of 2 x 64 predecessors, then windows of 6 x 64: the
+      // published-prefix front of a long scan advances up to 384 tiles per
+      // round trip once the nearest ones are found not to carry it (measured
+      // on 2^28 Int32 / Float32 and 2^27 Int64 / Float64 scanl1, us: first
+      // and later windows of 3 x 64 406 / 404 / 459 / 460; 2 then 6 394 /
+      // 399 / 422 / 413; 2 then 8 394 / 401 / 425 / 423; 2 then 3 or 4 410
+      // (Int32); 1 then 3..8 453-460; 4 then 4 425; 6 then 6 491; 8 then 8
+      // 500: the larger windows cost registers and occupancy)
 #ifndef ACC_LB_C0
-#define ACC_LB_C0 3
-#define ACC_LB_C1 3
+#define ACC_LB_C0 2
+#define ACC_LB_C1 6
 #endif
       X = wave_lookback<T, Op, ACC_LB_C0, ACC_LB_C1>(a.status, t, epoch, op, a.first);
     }

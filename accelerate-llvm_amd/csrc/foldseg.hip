@@ -239,7 +239,7 @@ __global__ __launch_bounds__(SP_BLOCK) void k_scan_plan(ScanPlanArgs<I> a) {
       if (lane == 0 && !last) Tagged<I>::publish(slot, agg, make_tag(epoch, ST_PREFIX));
     } else {
       if (lane == 0) Tagged<I>::publish(slot, agg, make_tag(epoch, ST_AGGREGATE));
-      X = wave_lookback<I>(a.status, j, epoch, add);
+      X = wave_lookback<I, OpAdd, 2, 2>(a.status, j, epoch, add);  // 128 predecessors per round trip
       if (lane == 0 && !last) Tagged<I>::publish(slot, add(X, agg), make_tag(epoch, ST_PREFIX));
     }
     if (lane == 0) s_prefix = X;

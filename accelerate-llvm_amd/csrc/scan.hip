@@ -35,8 +35,26 @@ template <typename T> struct ScanArgs {
 // predecessors have started no later than it did.  (Taking the next ticket
 // ahead of time lets consecutive tiles start up to a whole tile apart, and
 // every look-back then waits for that skew.)
+// Minimum waves per SIMD asked of the persistent kernel.  Left-to-right
+// Float32 (+) and (*) otherwise take 169 VGPRs (2 waves/SIMD); at 3 they
+// spill only the seed, reloaded on the first tile (2^28 Float32 scanl1:
+// 400 -> 394 us).  Elsewhere the default: Int32/Word32 already fit 3 waves,
+// and Float32 min/max, right-to-left Float32 and the 8-byte types would
+// spill inside the tile loop (2^27 Int64 424 -> 453 us, Float64 417 ->
+// 599 us at 3 waves).  The look-back stays a called function: inlined it
+// saves 10-12 VGPRs but measured slower (2^28 Int32 394 -> 403 us, Float32
+// at 3 waves 405 us).
+template <typename T, typename Op, bool REVERSE> struct ScanOccupancy {
+  static constexpr int MIN_WAVES =
+      std::is_same<T, float>::value && !REVERSE &&
+              (std::is_same<Op, OpAdd>::value || std::is_same<Op, OpMul>::value)
+          ? 3
+          : 1;
+};
+
 template <typename T, typename Op, int BLOCK, int K, bool REVERSE, bool ALIGNED>
-__global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs<T> a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ScanOccupancy<T, Op, REVERSE>::MIN_WAVES)))
+void k_scan(ScanArgs<T> a) {
   constexpr int VEC = Vec<T>::N;
   constexpr int TILE = BLOCK * K * VEC;
   __shared__ uint64_t s_tick;

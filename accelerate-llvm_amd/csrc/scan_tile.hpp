@@ -139,7 +139,10 @@ __device__ __forceinline__ void scan_tile(const TileIO<T>& a, Op op, uint32_t t,
       // and later windows of 3 x 64 406 / 404 / 459 / 460; 2 then 6 394 /
       // 399 / 422 / 413; 2 then 8 394 / 401 / 425 / 423; 2 then 3 or 4 410
       // (Int32); 1 then 3..8 453-460; 4 then 4 425; 6 then 6 491; 8 then 8
-      // 500: the larger windows cost registers and occupancy)
+      // 500: the larger windows cost registers and occupancy; at the same
+      // registers a wider first window is still slower: 3 then 6 402 / 402 /
+      // 473 / 467, 3 then 8 403 / 400 / 469 / 463, 4 then 6 or 8 422 / 420 /
+      // 497 / 494)
 #ifndef ACC_LB_C0
 #define ACC_LB_C0 2
 #define ACC_LB_C1 6

@@ -257,8 +257,8 @@ int stencil_launch(acc_stream_t s, const T* w, int boundary, const T* cval, cons
   const int taps = (2 * RY + 1) * (2 * RX + 1);
   for (int i = 0; i < 81; ++i) a.wt[i] = i < taps ? w[i] : T();
   const int64_t nt = tiles_x * tiles_y;
-  const int64_t ni = (ix1 - ix0) * (iy1 - iy0);
-  // one launch: blocks [0, ni) take the interior tiles, the rest the frame
+  // one launch: blocks [0, (ix1-ix0)*(iy1-iy0)) take the interior tiles, the
+  // rest the frame
   hipLaunchKernelGGL((k_stencil2d<T, RY, RX, 2>), dim3((unsigned)nt), dim3(ST_BLOCK), 0, s->stream, a);
   ACC_HIP_CHECK(hipGetLastError());
   return ACC_SUCCESS;

@@ -43,11 +43,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def timed(stream, fn, reps=20, warmup=2, stat="median"):
+def timed(stream, fn, reps=20, warmup=2, stat="median", warm_ms=20.0):
     """Median (or mean) milliseconds of `reps` calls of fn() on `stream`, each
     bracketed by its own pair of HIP events on that stream (calls stay back to
-    back)."""
+    back).  Before them: `warmup` calls, one timed call, then as many more as
+    make about `warm_ms` of device time (at most 500).  The clocks ramp up over
+    several milliseconds of work after an idle pause, such as the host-side
+    setup between workloads (8192^2 Float32 stencils: 92-106 us with two
+    warm-up calls, still up to 103 us after 5 ms, 85-89 us after 18 ms), so
+    the timed calls see the steady state."""
     for _ in range(warmup):
+        fn()
+    e0, e1 = A.Event(stream.device), A.Event(stream.device)
+    e0.record(stream)
+    fn()
+    e1.record(stream)
+    e1.synchronize()
+    per_ms = max(e0.elapsed_ms(e1), 1e-3)
+    for _ in range(min(500, int(warm_ms / per_ms))):
         fn()
     ev = [A.Event(stream.device) for _ in range(reps + 1)]
     stream.synchronize()

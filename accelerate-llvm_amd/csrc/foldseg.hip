@@ -333,279 +333,50 @@ __global__ __launch_bounds__(BLOCK) void k_fold_seg(SegArgs<T, Starts> a) {
   __shared__ int64_t s_open_g;
   __shared__ T s_open_v;
   __shared__ __align__(16) T s_data[BLOCK * RS];
-  SegOp<Op> sop{Op{}};
-  Op op{};
-  const T id = identity<T>(op);
-  const S sid{id, 0u};  // identity of the segmented combine
-  const int lane = lane_id();
-  const int w = threadIdx.x / 64;
   const uint32_t t = blockIdx.x;
-  const int64_t base = (int64_t)t * TILE;
-  // position (in the tile) of vector kk of this lane in half h
-  auto vpos = [&](int h, int kk) -> int { return h * HALF + ((w * KH + kk) * 64 + lane) * VEC; };
+#include "foldseg_tile.inc"
+}
 
-  // ---- every wave fetches the tile's segment range (wave 0 also its first
-  // SLOT starts), then issues its data loads: one dwordx4 per vector, so the
-  // loads stay straight-line and each later wait names exactly the loads it
-  // needs.  A vector outside the row re-reads the block holding the row's
-  // last element (an aligned 16-byte block with a valid element never
-  // crosses a page); the run below turns padding into the identity.  The
-  // staging overlaps the loads (barriers wait only for LDS traffic).
-  int64_t f0 = 0, f1 = 0;
-  int32_t my_rel = 0;
-  if constexpr (!UNIFORM) {
-    // (clamped: offsets that are not ascending give wrong results, never
-    // accesses outside the offsets)
-    f0 = min(max(a.st.first[t], (int64_t)0), a.nseg);
-    f1 = min(max(a.st.first[t + 1], f0), a.nseg);
-    my_rel = a.st.slots[(size_t)t * SLOT + lane];
-  }
-  T v[K][VEC];
-  const int64_t blk_hi = (a.e_end - 1) / VEC * VEC;  // block of the row's last element
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int64_t e0 = min(base + vpos(k / KH, k % KH), blk_hi);
-    const Vec<T> r = load16_stream(a.in + e0);
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) v[k][j] = r.e[j];
-  }
-
-  // ---- segment heads: every start in [base, base + TILE] sets its bit.
-  // Segment g in [0, S) covers [start(g), start(g+1)); "segment" S collects
-  // the elements after the last one and elements before start(0) belong to
-  // "segment" -1 (no results).
-  int64_t g_lo, cnt;
-  if (threadIdx.x == 0) s_open_g = -1;  // no open segment end yet
-  if constexpr (UNIFORM) {
-    for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
-    g_lo = max(base - a.st.shift, (int64_t)0) / a.st.len;
-    const int64_t g_hi = min(max(base + TILE - a.st.shift, (int64_t)0) / a.st.len, a.nseg);
-    cnt = g_hi - g_lo + 2;
-    __syncthreads();
-    for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
-      const int64_t p = a.st.start(g_lo + i) - base;
-      if (g_lo + i <= a.nseg && p >= 0 && p <= TILE) atomicOr(&s_heads[p >> 5], 1u << (p & 31));
-    }
-    __syncthreads();
+// The tiles of two foldSegs over one tile plan in one launch: workgroups
+// [0, n0) take the first fold's tiles, the rest the second's, so the second
+// fold's first tiles fill the CUs that the first fold's last ones leave idle
+// (one launch tail instead of two).  Both share the LDS declared here.
+template <typename T, typename Op0, typename Op1, typename Starts, int BLOCK, int K, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_fold_seg_pair(SegArgs<T, Starts> a0, SegArgs<T, Starts> a1, uint32_t n0) {
+  constexpr bool UNIFORM = std::is_same<Starts, UniformStarts>::value;
+  constexpr int VEC = Vec<T>::N;
+  constexpr int W = BLOCK / 64;
+  constexpr int TILE = BLOCK * K * VEC;
+  constexpr int NBW = TILE / 32 + 1;  // head bitmap: tile positions 0..TILE
+#ifndef ACC_SEG_KH
+#define ACC_SEG_KH 4
+#endif
+  constexpr int KH = ACC_SEG_KH;      // vectors per thread per part
+  constexpr int NH = K / KH;          // halves per tile
+  constexpr int R = KH * VEC;         // run: elements per thread per half
+  constexpr int HALF = BLOCK * R;
+  constexpr int RS = R + 16 / (int)sizeof(T);  // LDS row stride of a run
+  static_assert(K % KH == 0 && R <= 16, "run layout");
+  static_assert(TILE + 1 < 32767, "tile-relative starts are 16-bit");
+  using S = SegV<T>;
+  __shared__ uint32_t s_heads[NBW];
+  __shared__ int16_t s_rel[UNIFORM ? 1 : CAP];  // tile-relative starts (-1 .. TILE+1)
+  __shared__ int16_t s_pre[UNIFORM ? 1 : NBW];  // heads in the bitmap words before each word
+  __shared__ int32_t s_nin, s_neg, s_mode;      // starts in [0, TILE]; one before 0; search mode
+  __shared__ S s_wave[W];
+  __shared__ int64_t s_open_g;
+  __shared__ T s_open_v;
+  __shared__ __align__(16) T s_data[BLOCK * RS];
+  if (blockIdx.x < n0) {
+    using Op = Op0;
+    const SegArgs<T, Starts>& a = a0;
+    const uint32_t t = blockIdx.x;
+#include "foldseg_tile.inc"
   } else {
-    g_lo = f0;
-    cnt = f1 - f0 + 2;  // >= 2
-    if (cnt <= SLOT) {
-      if (w == 0) {
-        for (int i = lane; i < NBW; i += 64) s_heads[i] = 0u;
-        const bool in = lane < cnt && my_rel >= 0 && my_rel <= TILE;
-        if (lane < cnt) {
-          s_rel[lane] = (int16_t)my_rel;
-          if (in) atomicOr(&s_heads[my_rel >> 5], 1u << (my_rel & 31));
-        }
-        const uint64_t inb = __ballot(in);
-        if (lane == 0) {
-          s_nin = __popcll(inb);
-          s_neg = my_rel < 0 ? 1 : 0;
-        }
-      }
-    } else {
-      // many segments start in this tile: stage them from the offsets
-      for (int i = threadIdx.x; i < NBW; i += BLOCK) s_heads[i] = 0u;
-      if (threadIdx.x == 0) s_nin = 0;
-      __syncthreads();
-      int my_in = 0;
-      for (int64_t i = threadIdx.x; i < cnt; i += BLOCK) {
-        const int64_t g = g_lo + i;
-        const int32_t r = g > a.nseg ? TILE + 1 : rel_pos(a.st.start(g), base, TILE);
-        if (cnt <= CAP) s_rel[i] = (int16_t)r;
-        if (r >= 0 && r <= TILE) {
-          atomicOr(&s_heads[r >> 5], 1u << (r & 31));
-          ++my_in;
-        }
-        if (i == 0) s_neg = r < 0 ? 1 : 0;
-      }
-      if (my_in) atomicAdd(&s_nin, my_in);
-      __syncthreads();
-    }
-    // wave 0: heads before each bitmap word.  When the starts inside the
-    // tile are distinct (no empty segment begins inside it: as many heads as
-    // starts in [0, TILE]) the segment of an end is a count of heads;
-    // otherwise a binary search over the starts, in LDS or, past CAP of
-    // them, in the offsets.
-    if (w == 0) {
-      constexpr int WPL = (NBW + 63) / 64;  // bitmap words per lane
-      int c[WPL];
-      int sum = 0;
-#pragma unroll
-      for (int k = 0; k < WPL; ++k) {
-        const int wd = lane * WPL + k;
-        c[k] = wd < NBW ? __popc(s_heads[wd]) : 0;
-        sum += c[k];
-      }
-      const int incl = wave_inclusive_scan(sum, OpAdd{});
-      int run = incl - sum;
-#pragma unroll
-      for (int k = 0; k < WPL; ++k) {
-        const int wd = lane * WPL + k;
-        if (wd < NBW) s_pre[wd] = (int16_t)run;
-        run += c[k];
-      }
-      if (lane == 63) s_mode = incl == s_nin ? 0 : (cnt <= CAP ? 1 : 2);
-    }
-    __syncthreads();
-  }
-  auto emit = [&](int64_t g, T r) {
-    if (g >= 0 && g < a.nseg) {
-      if (a.has_seed) r = op(a.seed, r);  // seed on the left
-      a.out[g] = r;
-    }
-  };
-  // padding positions (outside the row) exist only in a partial tile
-  const bool full = base >= a.e_lo && base + TILE <= a.e_end;
-
-  // The halves, in one copy per way of finding the segment of an end: a
-  // count of heads (MODE 0), a binary search over the starts in LDS (1) or
-  // in the offsets (2).  Separate copies keep the offsets' loads out of the
-  // others, where a wait on them would also wait for the data loads still in
-  // flight.
-  const int neg = UNIFORM ? 0 : s_neg;
-  auto halves = [&](auto mode) __attribute__((always_inline)) {
-    constexpr int MODE = decltype(mode)::value;
-    constexpr bool HBM = MODE == 2;
-    // the segment whose last element is at tile position p (0 <= p < TILE)
-    auto seg_ending_at = [&](int p) -> int64_t {
-      if constexpr (UNIFORM) {
-        return (base + p - a.st.shift) / a.st.len;
-      } else if constexpr (MODE == 0) {
-        // starts at or before p: the one before the tile (if any) and the
-        // heads at 0 .. p
-        const int wd = p >> 5;
-        const int h = s_pre[wd] + __popc(s_heads[wd] & (0xffffffffu >> (31 - (p & 31))));
-        return g_lo - 1 + neg + h;
-      } else {
-        // smallest index i with start(g_lo + i) > p, minus one
-        int64_t x = 0, y = cnt;
-        while (x < y) {
-          const int64_t m = (x + y) >> 1;
-          int64_t sm;
-          if constexpr (HBM) sm = g_lo + m > a.nseg ? (int64_t)TILE + 1 : a.st.start(g_lo + m) - base;
-          else sm = (int64_t)s_rel[m];
-          if (sm <= p) x = m + 1; else y = m;
-        }
-        return g_lo + x - 1;
-      }
-    };
-    S carry = sid;  // segmented combination of the tile's earlier halves
-#pragma unroll
-    for (int h = 0; h < NH; ++h) {
-      // ---- transpose the half through LDS: vector at run r, offset o
-#pragma unroll
-      for (int kk = 0; kk < KH; ++kk) {
-        const int q0 = vpos(0, kk);  // position within the half
-        const int r = q0 / R, o = q0 - r * R;
-        Vec<T> vv;
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) vv.e[j] = v[h * KH + kk][j];
-        *(Vec<T>*)&s_data[r * RS + o] = vv;
-      }
-      __syncthreads();
-      T x[R];
-#pragma unroll
-      for (int q = 0; q < R / VEC; ++q) {
-        const Vec<T> vv = *(const Vec<T>*)&s_data[threadIdx.x * RS + q * VEC];
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) x[q * VEC + j] = vv.e[j];
-      }
-      // head and end bits of the run
-      const int pos0 = h * HALF + threadIdx.x * R;
-      const uint64_t word = (uint64_t)s_heads[pos0 >> 5] | ((uint64_t)s_heads[(pos0 >> 5) + 1] << 32);
-      const uint32_t mask = (1u << R) - 1u;
-      uint32_t hb = (uint32_t)(word >> (pos0 & 31)) & mask;
-      uint32_t eb = (uint32_t)(word >> ((pos0 & 31) + 1)) & mask;
-      if (!full) {
-        // padding: the identity, each position a head of its own dead segment
-        uint32_t invalid = 0;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-          const int64_t e = base + pos0 + i;
-          const bool bad = !(e >= a.e_lo && e < a.e_end);
-          invalid |= (bad ? 1u : 0u) << i;
-          x[i] = bad ? id : x[i];
-        }
-        hb |= invalid;
-        eb &= ~invalid;
-      }
-      // ---- sequential segmented fold of the run; ends after a head in the
-      // run are final; the first end before any head ("run-open") waits for
-      // the prefix of the runs before it.  A half holds one or two segment
-      // boundaries per wave on average, so each position first tests the
-      // wave's own head and end bits (one OR over the wave, scalar branches):
-      // a position with no head in the wave costs the combine alone, one
-      // with no end skips the end handling.
-      static_assert(R <= 16, "head and end bits share one word");
-      const uint32_t wbits = wave_reduce(hb | eb << 16, OpOr{});
-      T acc = id;
-      int64_t ro_g = -1;
-      T ro_v = id;
-#pragma unroll
-      for (int i = 0; i < R; ++i) {
-        if (wbits >> i & 1u) acc = (hb >> i & 1u) ? x[i] : op(acc, x[i]);
-        else acc = op(acc, x[i]);
-        if (wbits >> (16 + i) & 1u) {
-          // (keeps this test a scalar branch: without it the two tests merge
-          // into one per-lane condition evaluated at every position)
-          __builtin_amdgcn_sched_barrier(0);
-          if (eb >> i & 1u) {
-            const int64_t g = seg_ending_at(pos0 + i);
-            if (hb & ((2u << i) - 1u)) {  // a head at or before i in the run
-              emit(g, acc);
-            } else {
-              ro_g = g;
-              ro_v = acc;
-            }
-          }
-        }
-      }
-      // ---- segmented exclusive scan of the run aggregates across the block
-      const S inc = wave_inclusive_scan(S{acc, hb != 0 ? 1u : 0u}, sop);
-      const S up = shift_up1(inc);  // all lanes take part in the cross-lane move
-      if (lane == 63) s_wave[w] = inc;
-      __syncthreads();
-      S wp = carry, tot = carry;
-#pragma unroll
-      for (int i = 0; i < W; ++i) {
-        const S x2 = s_wave[i];
-        if (i < w) wp = sop(wp, x2);
-        tot = sop(tot, x2);
-      }
-      const S P = sop(wp, lane > 0 ? up : sid);  // everything in the tile before this run
-      if (ro_g >= 0) {
-        const T r = op(P.v, ro_v);
-        if (P.f) {
-          emit(ro_g, r);
-        } else {  // no head in the tile before it: the tile's one open end
-          s_open_g = ro_g;
-          s_open_v = r;
-        }
-      }
-      carry = tot;
-      __syncthreads();  // s_data and s_wave are reused by the next half
-    }
-    return carry;
-  };
-  S carry;
-  if constexpr (UNIFORM) {
-    carry = halves(std::integral_constant<int, 0>{});
-  } else {
-    const int m = s_mode;
-    if (m == 0) carry = halves(std::integral_constant<int, 0>{});
-    else if (m == 1) carry = halves(std::integral_constant<int, 1>{});
-    else carry = halves(std::integral_constant<int, 2>{});
-  }
-
-  // ---- the tile's record for the fix-up: aggregate and open end
-  if (threadIdx.x == 0) {
-    TileCarry<T> c;
-    c.agg = carry;
-    c.open_g = s_open_g;
-    c.open_v = c.open_g >= 0 ? s_open_v : id;
-    a.carry.put(t, c);
+    using Op = Op1;
+    const SegArgs<T, Starts>& a = a1;
+    const uint32_t t = blockIdx.x - n0;
+#include "foldseg_tile.inc"
   }
 }
 
@@ -1123,6 +894,19 @@ extern "C" int acc_hip_fold_seg_planned_multi(acc_stream_t s, acc_segplan_t plan
     ACC_CHECK(stream_workspace(s, (size_t)count * nfix * Tagged<SegV<T>>::NW * 8, &ctl, &status));
     ACC_CHECK(stream_scratch(s, (size_t)count * carry_bytes, &scr));
     MultiFix<T> m{};
+    // two Float32 folds, each (+) or max (the common pair: a segment's sum and
+    // maximum): their tiles in one k_fold_seg_pair launch
+    const bool pair = std::is_same<T, float>::value && count == 2 && 2 * ntiles <= (int64_t)UINT32_MAX &&
+                      (ops[0] == ACC_OP_ADD || ops[0] == ACC_OP_MAX) && (ops[1] == ACC_OP_ADD || ops[1] == ACC_OP_MAX);
+    auto fold_args = [&](int i, auto st) {
+      using Starts = decltype(st);
+      st.first = tp.first;
+      st.slots = tp.slots;
+      st.shift = plan->shift;
+      const T* in0 = (const T*)ins[i] - plan->shift;
+      return SegArgs<T, Starts>{in0, m.out[i], st, nseg, plan->shift, n + plan->shift, (uint32_t)ntiles,
+                                m.has_seed[i], m.seed[i], m.carry[i]};
+    };
     for (int i = 0; i < count; ++i) {
       char* cb = (char*)scr + (size_t)i * carry_bytes;
       m.carry[i] = CarryArrays<T>{(T*)cb, (uint8_t*)(cb + cv_bytes), (T*)(cb + cv_bytes + cf_bytes),
@@ -1131,17 +915,13 @@ extern "C" int acc_hip_fold_seg_planned_multi(acc_stream_t s, acc_segplan_t plan
       m.has_seed[i] = seeds && seeds[i] ? 1u : 0u;
       m.seed[i] = m.has_seed[i] ? *(const T*)seeds[i] : T();
       m.op[i] = (int32_t)ops[i];
-      const T* in0 = (const T*)ins[i] - plan->shift;
+      if (m.has_seed[i]) ACC_CHECK(launch_fill_empty(s, offsets, I, nseg, m.out[i], m.seed[i]));
+      if (pair) continue;  // both folds' tiles in one launch below
       const int rc = with_op<T>(ops[i], [&](auto f) {
         using Op = decltype(f);
-        if (m.has_seed[i]) ACC_CHECK(launch_fill_empty(s, offsets, I, nseg, m.out[i], m.seed[i]));
         auto launch = [&](auto st) {
           using Starts = decltype(st);
-          st.first = tp.first;
-          st.slots = tp.slots;
-          st.shift = plan->shift;
-          SegArgs<T, Starts> a{in0, m.out[i], st, nseg, plan->shift, n + plan->shift, (uint32_t)ntiles, m.has_seed[i],
-                               m.seed[i], m.carry[i]};
+          const SegArgs<T, Starts> a = fold_args(i, st);
           hipLaunchKernelGGL((k_fold_seg<T, Op, Starts, BLOCK, K, CAP>), dim3((unsigned)ntiles), dim3(BLOCK), 0,
                              s->stream, a);
           ACC_HIP_CHECK(hipGetLastError());
@@ -1151,6 +931,25 @@ extern "C" int acc_hip_fold_seg_planned_multi(acc_stream_t s, acc_segplan_t plan
         return launch(OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr, nullptr, 0});
       });
       if (rc) return rc;
+    }
+    if constexpr (std::is_same<T, float>::value) {
+      if (pair) {
+        auto launch = [&](auto st) {
+          using Starts = decltype(st);
+          const SegArgs<T, Starts> a0 = fold_args(0, st), a1 = fold_args(1, st);
+          auto go = [&](auto o0, auto o1) {
+            hipLaunchKernelGGL((k_fold_seg_pair<T, decltype(o0), decltype(o1), Starts, BLOCK, K, CAP>),
+                               dim3((unsigned)(2 * ntiles)), dim3(BLOCK), 0, s->stream, a0, a1, (uint32_t)ntiles);
+            ACC_HIP_CHECK(hipGetLastError());
+            return (int)ACC_SUCCESS;
+          };
+          if (ops[0] == ACC_OP_ADD) return ops[1] == ACC_OP_ADD ? go(OpAdd{}, OpAdd{}) : go(OpAdd{}, OpMax{});
+          return ops[1] == ACC_OP_ADD ? go(OpMax{}, OpAdd{}) : go(OpMax{}, OpMax{});
+        };
+        const int rc = I == 4 ? launch(OffsetStarts<int32_t>{(const int32_t*)offsets, nullptr, nullptr, 0})
+                              : launch(OffsetStarts<int64_t>{(const int64_t*)offsets, nullptr, nullptr, 0});
+        if (rc) return rc;
+      }
     }
     if (ntiles > 1) {
       hipLaunchKernelGGL((k_seg_fix_multi<T, FIX>), dim3((unsigned)(count * nfix)), dim3(FIX), 0, s->stream, m,

@@ -264,6 +264,47 @@ def test_several_folds_in_one_call(stream, dt):
     plan.destroy()
 
 
+@pytest.mark.parametrize("ops", [(A.ADD, A.MAX), (A.MAX, A.ADD), (A.ADD, A.ADD), (A.MAX, A.MAX)])
+@pytest.mark.parametrize("offs", [np.int32, np.int64])
+def test_two_float32_folds_in_one_launch(stream, ops, offs):
+    """Two Float32 folds of (+) or max in one multi call run both folds' tiles
+    in one launch (k_fold_seg_pair); each result is bit-identical to its own
+    planned call, with and without a seed, empty segments included."""
+    rng = np.random.default_rng(31 + 2 * ops[0] + ops[1] + (offs == np.int64))
+    n, nseg = 1_000_003, 3001
+    off = segments(rng, n, nseg, empties=True).astype(offs)
+    xs = [data(rng, np.float32, n) for _ in range(2)]
+    ds = [A.DeviceArray.from_numpy(stream, x) for x in xs]
+    d_off = A.DeviceArray.from_numpy(stream, off)
+    plan = A.SegPlan(0)
+    plan.build(stream, np.float32, ds[0], n, d_off, nseg)
+    spec = [(ops[0], None), (ops[1], np.float32(2.5))]
+    outs = [A.DeviceArray.from_numpy(stream, np.full(nseg, 7, np.float32)) for _ in spec]
+    plan.fold_seg_multi(stream, [(op, sd, d, o) for (op, sd), d, o in zip(spec, ds, outs)])
+    for (op, sd), d, o, x in zip(spec, ds, outs, xs):
+        got = o.to_numpy(stream)
+        assert np.array_equal(got, planned(stream, plan, op, sd, d, 1, nseg, np.float32)), (op, sd)
+        against_oracle(op, sd, x, off.astype(np.int64), got)
+    plan.destroy()
+
+
+def test_two_float32_folds_in_one_launch_at_c2_size(stream):
+    """The benchmark step's folds (+) and max over 2^26 Float32 values in 2^16
+    segments through one multi call: each equals its own planned call."""
+    from acc_hip import workloads as W
+    lens = W.segment_lengths(42, 1 << 16, 16384, 1 << 26)
+    x = W.normal_f32(42, 1 << 26)
+    d, d_len = A.DeviceArray.from_numpy(stream, x), A.DeviceArray.from_numpy(stream, lens)
+    d_off = A.DeviceArray(lens.size + 1, np.int64)
+    plan = A.SegPlan(0)
+    plan.build_lengths(stream, np.float32, d, x.size, d_len, lens.size, d_off)
+    o_sum, o_max = A.DeviceArray(lens.size, np.float32), A.DeviceArray(lens.size, np.float32)
+    plan.fold_seg_multi(stream, [(A.ADD, None, d, o_sum), (A.MAX, None, d, o_max)])
+    assert np.array_equal(o_sum.to_numpy(stream), planned(stream, plan, A.ADD, None, d, 1, lens.size, np.float32))
+    assert np.array_equal(o_max.to_numpy(stream), planned(stream, plan, A.MAX, None, d, 1, lens.size, np.float32))
+    plan.destroy()
+
+
 def test_several_folds_limits(stream):
     rng = np.random.default_rng(23)
     plan = A.SegPlan(0)

@@ -19,7 +19,9 @@
 //   k_fold_seg  one workgroup per tile: every segment end whose segment
 //               starts inside the tile is final at once and written; the one
 //               end that continues a segment from earlier tiles ("open") and
-//               the tile's segmented aggregate go to a per-tile record;
+//               the tile's segmented aggregate go to a per-tile record
+//               (k_fold_seg_pair: the tiles of two folds over one plan in
+//               one launch, the same tile body);
 //   k_seg_fix   a segmented scan over the tile aggregates (256 tiles per
 //               workgroup, chained by a look-back), which completes the open
 //               ends (at most one per tile).  (One workgroup over all the

@@ -724,7 +724,7 @@ def main():
         "config": {
             "workload": "Float32 fold1Seg (+) and fold1Seg max over 2^26 N(0,1) values in 2^16 segments "
                         "(lengths log-uniform in [1,16384], sum 2^26) + Int64 scanl (+) 0 of the lengths "
-                        "(the offsets and their tile plan in one launch per step; the two folds in one call on that plan)",
+                        "(the offsets and their tile plan in one launch per step; the two folds in one call on that plan, their tiles in one launch)",
             "elements": N_VALUES,
             "segments": N_SEG,
             "seed": SEED,

@@ -258,8 +258,10 @@ int acc_hip_fold_seg_planned(acc_stream_t s, acc_segplan_t plan, acc_type t, acc
 /* Several foldSegs (rows of one, outer = 1) over one plan in one call: fold i
  * combines ins[i] with ops[i] (seeds[i] or NULL; seeds itself may be NULL)
  * into outs[i], 1 <= count <= 8, every input of the plan's type and
- * alignment.  The tile kernels run one after the other and the fix-ups of all
- * the folds share one launch. */
+ * alignment.  The tile kernels run one after the other (two Float32 folds,
+ * each ACC_OP_ADD or ACC_OP_MAX: both folds' tiles in one launch) and the
+ * fix-ups of all the folds share one launch; every result is bit-identical to
+ * the fold's own acc_hip_fold_seg_planned call. */
 int acc_hip_fold_seg_planned_multi(acc_stream_t s, acc_segplan_t plan, acc_type t, int count, const acc_op* ops,
                                    const void* const* seeds, const void* const* ins, void* const* outs);
 int acc_hip_seg_plan_destroy(acc_segplan_t plan);
